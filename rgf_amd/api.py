@@ -1,0 +1,392 @@
+"""The reference's MATLAB interface over librgf.
+
+``cal_smatrix_RGF``, ``cal_smatrix_RGF_3D``, ``setup_channels``,
+``setup_channels_3D`` and ``build_laplacian_1d`` take the same arguments and
+return the same things as the MATLAB functions of chiaweihsu/RGF
+(src/cal_smatrix_RGF.m:1, src/cal_smatrix_RGF_3D.m:1, src/setup_channels.m:1,
+src/setup_channels_3D.m:1, src/build_laplacian_1d.m:1), with Python types:
+``syst`` is a dict, a MATLAB cell is a list of strings, a MATLAB struct is a
+dict, matrices are numpy arrays indexed like MATLAB (ny x nx in 2D,
+nx x ny x nz in 3D) and channel indices stay 1-based.  ``in`` is a Python
+keyword, so that argument is spelled ``in_``.
+
+All numerical work runs in librgf on the GPU; this module only marshals.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib as L
+
+_BC_NAMES = {
+    "dirichlet": L.RGF_BC_DIRICHLET, "pec": L.RGF_BC_DIRICHLET,
+    "neumann": L.RGF_BC_NEUMANN, "pmc": L.RGF_BC_NEUMANN,
+    "dirichletneumann": L.RGF_BC_DIRICHLET_NEUMANN, "pecpmc": L.RGF_BC_DIRICHLET_NEUMANN,
+    "neumanndirichlet": L.RGF_BC_NEUMANN_DIRICHLET, "pmcpec": L.RGF_BC_NEUMANN_DIRICHLET,
+}
+_PRECISIONS = {"fp64": L.RGF_PRECISION_FP64, "fp32_refined": L.RGF_PRECISION_FP32_REFINED,
+               "mixed": L.RGF_PRECISION_FP32_REFINED}
+
+
+def _is_num(x) -> bool:
+    return isinstance(x, (int, float, complex, np.number)) and not isinstance(x, bool)
+
+
+def _bc_1d(BC) -> tuple[int, float]:
+    """setup_channels.m:102-115: numeric -> Bloch ka; 'periodic' -> Bloch 0."""
+    if _is_num(BC) or (isinstance(BC, np.ndarray) and BC.size == 1):
+        ka = complex(np.asarray(BC).ravel()[0]) if isinstance(BC, np.ndarray) else complex(BC)
+        if ka.imag != 0:
+            raise ValueError(f"k*a = {ka.real:g} + 1i*{ka.imag:g} is a complex number; has to be real for a "
+                             "complete orthonormal transverse basis.")
+        return L.RGF_BC_BLOCH, ka.real
+    if not isinstance(BC, str):
+        raise ValueError("Boundary condition BC has to be a character vector or string, or numeric scalar "
+                         "(for Bloch periodic BC)")
+    low = BC.lower()
+    if low == "bloch":
+        raise ValueError("To use Bloch periodic boundary condition, set BC to k*a where k is the Bloch wave "
+                         "number and a is the periodicity")
+    if low == "periodic":
+        return L.RGF_BC_BLOCH, 0.0
+    if low in _BC_NAMES:
+        return _BC_NAMES[low], 0.0
+    raise ValueError(f"Boundary condition BC = {BC} is not an eligible option")
+
+
+def _bc_xy(BC) -> tuple[list[int], list[float]]:
+    """setup_channels_3D.m:387-405: a string for both directions or [kxa, kya]."""
+    if isinstance(BC, str):
+        low = BC.lower()
+        if low == "bloch":
+            raise ValueError("To use Bloch periodic boundary condition, set BC to [kx*ax, ky*ay] where (kx,ky) is "
+                             "the Bloch wave vector and (ax,ay) is the periodicity")
+        if low == "periodic":
+            return [L.RGF_BC_BLOCH] * 2, [0.0, 0.0]
+        if low in ("dirichlet", "pec"):
+            return [L.RGF_BC_DIRICHLET] * 2, [0.0, 0.0]
+        raise ValueError(f"Boundary condition BC = {BC} is not an eligible option")
+    v = np.asarray(BC).ravel()
+    if v.size != 2 or not np.issubdtype(v.dtype, np.number):
+        raise ValueError("Boundary condition BC has to be a character vector or string, or numeric vector with 2 "
+                         "elements (for Bloch periodic BC)")
+    for lab, x in (("kx", v[0]), ("ky", v[1])):
+        if np.imag(x) != 0:
+            raise ValueError(f"{lab}*a = {np.real(x):g} + 1i*{np.imag(x):g} is a complex number; has to be real "
+                             "for a complete orthonormal transverse basis.")
+    return [L.RGF_BC_BLOCH] * 2, [float(np.real(v[0])), float(np.real(v[1]))]
+
+
+def _scalar(x, name) -> complex:
+    if not (_is_num(x) or (isinstance(x, np.ndarray) and x.size == 1)):
+        raise ValueError(f"{name} must be a numeric scalar")
+    return complex(np.asarray(x).ravel()[0])
+
+
+class _Prob:
+    """Owns the numpy buffers behind an rgf_problem."""
+
+    def __init__(self, ndim, dims, eps, epsL, epsR, k0dx, bcs, kas):
+        self.keep = []
+        p = L.Problem()
+        p.ndim = ndim
+        p.dims[:] = list(dims)
+        if eps is not None and eps.size:
+            # column-major flattening: slices (columns in 2D, z-planes in 3D) are contiguous
+            re = np.ascontiguousarray(np.real(eps).ravel(order="F"), dtype=np.float64)
+            self.keep.append(re)
+            p.eps_re = re.ctypes.data
+            if np.iscomplexobj(eps):
+                im = np.ascontiguousarray(np.imag(eps).ravel(order="F"), dtype=np.float64)
+                self.keep.append(im)
+                p.eps_im = im.ctypes.data
+        for fld, v in (("eps_L", epsL), ("eps_R", epsR), ("k0dx", k0dx)):
+            getattr(p, fld)[:] = [v.real, v.imag]
+        p.bc[:] = list(bcs) + [0] * (2 - len(bcs))
+        p.bloch_ka[:] = list(kas) + [0.0] * (2 - len(kas))
+        p.eps_on_device = 0
+        self.p = p
+
+
+def _problem_2d(syst, need_eps=True) -> _Prob:
+    if not isinstance(syst, dict):
+        raise ValueError("syst must be a structure")
+    for key in ("epsilon", "epsilon_L", "epsilon_R", "k0dx", "yBC"):
+        if key not in syst:
+            raise ValueError(f"syst.{key} must be provided")
+    eps = np.asarray(syst["epsilon"])
+    if eps.ndim == 1:
+        eps = eps[:, None]
+    if eps.ndim != 2 or not np.issubdtype(eps.dtype, np.number):
+        raise ValueError("syst.epsilon must be a numeric matrix")
+    ny, nx = eps.shape
+    if ny == 0:
+        raise ValueError("need at least one site along the transverse (y) direction")
+    bc, ka = _bc_1d(syst["yBC"])
+    return _Prob(2, (ny, nx, 1), eps, _scalar(syst["epsilon_L"], "syst.epsilon_L"),
+                 _scalar(syst["epsilon_R"], "syst.epsilon_R"), _scalar(syst["k0dx"], "syst.k0dx"), [bc], [ka])
+
+
+def _problem_3d(syst) -> _Prob:
+    if not isinstance(syst, dict):
+        raise ValueError("syst must be a structure")
+    for key in ("epsilon", "epsilon_L", "epsilon_R", "k0dx", "xyBC"):
+        if key not in syst:
+            raise ValueError(f"syst.{key} must be provided")
+    eps = np.asarray(syst["epsilon"])
+    if eps.ndim == 2:
+        eps = eps[:, :, None]
+    if eps.ndim != 3 or not np.issubdtype(eps.dtype, np.number):
+        raise ValueError("syst.epsilon must be a 3D array")
+    nx, ny, nz = eps.shape
+    if nx * ny == 0:
+        raise ValueError("need at least one site along the transverse directions (x,y)")
+    bcs, kas = _bc_xy(syst["xyBC"])
+    return _Prob(3, (nx, ny, nz), eps, _scalar(syst["epsilon_L"], "syst.epsilon_L"),
+                 _scalar(syst["epsilon_R"], "syst.epsilon_R"), _scalar(syst["k0dx"], "syst.k0dx"), bcs, kas)
+
+
+def _query(prob: _Prob) -> tuple[int, int]:
+    lib = L.load_library()
+    nL, nR = C.c_int32(), C.c_int32()
+    L.check(lib.rgf_query(C.byref(prob.p), None, C.byref(nL), C.byref(nR), None, None))
+    return nL.value, nR.value
+
+
+def _parse_selection(spec, kind: str, nprop: tuple[int, int]):
+    """cal_smatrix_RGF.m:103-175 -> (L, R) where each is RGF_ALL or an int32 array."""
+    sides = [np.zeros(0, np.int32), np.zeros(0, np.int32)]
+    if isinstance(spec, dict):
+        for s, side in enumerate(("L", "R")):
+            key = f"ind_{kind}_{side}"
+            if key in spec:
+                v = np.asarray(spec[key]).ravel()
+                ok = v.size == 0 or (np.issubdtype(v.dtype, np.number) and np.all(np.isreal(v))
+                                     and np.all(np.round(np.real(v)) == np.real(v))
+                                     and np.real(v).min() > 0 and np.real(v).max() <= nprop[s])
+                if not ok:
+                    raise ValueError(f"{kind}.{key}, when specified, must be an array of positive integers not "
+                                     f"exceeding N_prop_{side}={nprop[s]}")
+                sides[s] = np.real(v).astype(np.int32)
+    elif isinstance(spec, (list, tuple)):
+        for k, el in enumerate(spec, start=1):
+            if not isinstance(el, str):
+                warnings.warn(f"element {kind}{{{k}}} has wrong format and is ignored; should be one of "
+                              "{'left', 'right', 'L', 'R'}")
+                continue
+            if el in ("left", "L"):
+                sides[0] = L.RGF_ALL
+            elif el in ("right", "R"):
+                sides[1] = L.RGF_ALL
+            else:
+                warnings.warn(f"element {kind}{{{k}}} = {el} ignored; should be one of {{'left', 'right', 'L', 'R'}}")
+    else:
+        raise ValueError(f"input argument '{kind}' must be a structure or a cell array")
+    return sides
+
+
+class _Sel:
+    def __init__(self, out_sides, in_sides):
+        self.keep = []
+        s = L.Selection()
+        for fld, v in (("in_L", in_sides[0]), ("in_R", in_sides[1]), ("out_L", out_sides[0]),
+                       ("out_R", out_sides[1])):
+            if isinstance(v, int):
+                setattr(s, "ind_" + fld, None)
+                setattr(s, "n_" + fld, v)
+            else:
+                a = np.ascontiguousarray(v, dtype=np.int32)
+                self.keep.append(a)
+                setattr(s, "ind_" + fld, L.iptr(a) if a.size else None)
+                setattr(s, "n_" + fld, int(a.size))
+        self.s = s
+
+
+def _options(precision="fp64", **kw) -> L.Options:
+    lib = L.load_library()
+    o = L.Options()
+    lib.rgf_default_options(C.byref(o))
+    if precision not in _PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+    o.precision = _PRECISIONS[precision]
+    for k, v in kw.items():
+        if v is not None:
+            setattr(o, k, v)
+    return o
+
+
+def _solve(prob: _Prob, out, in_, precision, device, opts):
+    lib = L.load_library()
+    nprop = _query(prob)
+    sel = _Sel(_parse_selection(out, "out", nprop), _parse_selection(in_, "in", nprop))
+    nout, nin = C.c_int32(), C.c_int32()
+    L.check(lib.rgf_query(C.byref(prob.p), C.byref(sel.s), None, None, C.byref(nout), C.byref(nin)))
+    if nout.value == 0 or nin.value == 0:
+        return np.zeros((nout.value, nin.value)), None      # cal_smatrix_RGF.m:181-184 (real zeros)
+    h = L.handle(device)
+    S = np.zeros((nout.value, nin.value), dtype=np.complex128, order="F")
+    info = L.Info()
+    o = _options(precision, **(opts or {}))
+    st = lib.rgf_solve(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp), C.byref(info))
+    L.check(st, h)
+    return S, info.as_dict()
+
+
+def cal_smatrix_RGF(syst: dict, out, in_, *, precision: str = "fp64", device: int = -1, options: dict | None = None,
+                    return_info: bool = False):
+    """[smatrix, channels] = cal_smatrix_RGF(syst, out, in) (cal_smatrix_RGF.m:1-291).
+
+    Unlike the reference's validation at cal_smatrix_RGF.m:89, a numeric
+    ``yBC`` (Bloch phase ky*a, as its help text :32-34 documents) is accepted.
+    ``precision`` is 'fp64' (complex FP64 throughout) or 'fp32_refined'.
+    """
+    prob = _problem_2d(syst)
+    S, info = _solve(prob, out, in_, precision, device, options)
+    ch = setup_channels(prob.p.dims[0], syst["yBC"], syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"])
+    return (S, ch, info) if return_info else (S, ch)
+
+
+def cal_smatrix_RGF_3D(syst: dict, out, in_, *, precision: str = "fp64", device: int = -1,
+                       options: dict | None = None, return_info: bool = False):
+    """[smatrix, channels] = cal_smatrix_RGF_3D(syst, out, in) (cal_smatrix_RGF_3D.m:1-306)."""
+    prob = _problem_3d(syst)
+    S, info = _solve(prob, out, in_, precision, device, options)
+    nx, ny = prob.p.dims[0], prob.p.dims[1]
+    ch = setup_channels_3D(nx, ny, syst["xyBC"], syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"])
+    return (S, ch, info) if return_info else (S, ch)
+
+
+def _channels_raw(prob: _Prob, N: int):
+    lib = L.load_library()
+    c = L.Channels()
+    arrs = {
+        "kt1": np.zeros(N), "kt2": np.zeros(N), "kl_L": np.zeros(2 * N), "kl_R": np.zeros(2 * N),
+        "sqrt_mu_L": np.zeros(2 * N), "sqrt_mu_R": np.zeros(2 * N),
+    }
+    iarrs = {k: np.zeros(N, np.int32) for k in ("ind_prop_L", "ind_prop_R", "ind_prop_conj_L", "ind_prop_conj_R")}
+    for k, a in arrs.items():
+        setattr(c, k, L.dptr(a))
+    for k, a in iarrs.items():
+        setattr(c, k, L.iptr(a))
+    L.check(lib.rgf_setup_channels(C.byref(prob.p), C.byref(c)))
+    return c, arrs, iarrs
+
+
+def _side(c, arrs, iarrs, s, conj: bool):
+    n = c.n_prop_L if s == "L" else c.n_prop_R
+    kl = arrs[f"kl_{s}"][0::2] + 1j * arrs[f"kl_{s}"][1::2]
+    mu = arrs[f"sqrt_mu_{s}"][0:2 * n:2] + 1j * arrs[f"sqrt_mu_{s}"][1:2 * n:2]
+    d = {"N_prop": n, "kxdx": kl, "ind_prop": iarrs[f"ind_prop_{s}"][:n].astype(np.int64),
+         "sqrt_mu": mu if np.any(mu.imag) else mu.real}
+    if conj and c.has_conj:
+        d["ind_prop_conj"] = iarrs[f"ind_prop_conj_{s}"][:n].astype(np.int64)
+    return d
+
+
+def setup_channels(ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
+    """channels = setup_channels(ny, BC, k0dx, epsilon_L, epsilon_R) (setup_channels.m:1-176).
+
+    ``fun_phi`` is a callable returning the ny x len(kydx) mode matrix.
+    """
+    if not (isinstance(ny, (int, np.integer)) and ny > 0):
+        raise ValueError("ny must be a positive integer scalar")
+    bc, ka = _bc_1d(BC)
+    two_sided = epsilon_R is not None and not (_is_num(epsilon_R) and np.isnan(epsilon_R))
+    eR = _scalar(epsilon_R, "epsilon_R") if two_sided else _scalar(epsilon_L, "epsilon_L")
+    prob = _Prob(2, (ny, 0, 1), None, _scalar(epsilon_L, "epsilon_L"), eR, _scalar(k0dx, "k0dx"), [bc], [ka])
+    c, arrs, iarrs = _channels_raw(prob, ny)
+    m = np.arange(1, ny + 1)[:, None]
+    if bc == L.RGF_BC_BLOCH:
+        def fun_phi(k):
+            return np.exp(1j * m * np.atleast_1d(k)[None, :]) / math.sqrt(ny)
+    elif bc == L.RGF_BC_DIRICHLET:
+        def fun_phi(k):
+            return np.sin(m * np.atleast_1d(k)[None, :]) * math.sqrt(2 / (ny + 1))
+    elif bc == L.RGF_BC_NEUMANN:
+        def fun_phi(k):
+            k = np.atleast_1d(k)[None, :]
+            return (np.cos((m - 0.5) * k) - (k == 0) * (1 - math.sqrt(0.5))) * math.sqrt(2 / ny)
+    elif bc == L.RGF_BC_DIRICHLET_NEUMANN:
+        def fun_phi(k):
+            return np.sin(m * np.atleast_1d(k)[None, :]) * math.sqrt(2 / (ny + 0.5))
+    else:
+        def fun_phi(k):
+            return np.cos((m - 0.5) * np.atleast_1d(k)[None, :]) * math.sqrt(2 / (ny + 0.5))
+    ch = {"kydx": arrs["kt1"].copy(), "fun_phi": fun_phi, "L": _side(c, arrs, iarrs, "L", True)}
+    if two_sided:
+        ch["R"] = ch["L"] if complex(epsilon_R) == complex(epsilon_L) else _side(c, arrs, iarrs, "R", True)
+    else:
+        ch["R"] = {"N_prop": float("nan")}
+    return ch
+
+
+def setup_channels_3D(nx: int, ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
+    """channels = setup_channels_3D(nx, ny, BC, k0dx, epsilon_L, epsilon_R) (setup_channels_3D.m:1-449)."""
+    for n, lab in ((nx, "nx"), (ny, "ny")):
+        if not (isinstance(n, (int, np.integer)) and n > 0):
+            raise ValueError(f"{lab} must be a positive integer scalar")
+    bcs, kas = _bc_xy(BC)
+    two_sided = epsilon_R is not None and not (_is_num(epsilon_R) and np.isnan(epsilon_R))
+    eR = _scalar(epsilon_R, "epsilon_R") if two_sided else _scalar(epsilon_L, "epsilon_L")
+    prob = _Prob(3, (nx, ny, 0), None, _scalar(epsilon_L, "epsilon_L"), eR, _scalar(k0dx, "k0dx"), bcs, kas)
+    N = nx * ny
+    c, arrs, iarrs = _channels_raw(prob, N)
+    mx = np.arange(1, nx + 1)[:, None]
+    my = np.arange(1, ny + 1)[None, :]
+    if bcs[0] == L.RGF_BC_BLOCH:
+        def fun_phi(kx, ky):
+            return np.exp(1j * mx * kx) * (np.exp(1j * my * ky) / math.sqrt(nx * ny))
+    else:
+        def fun_phi(kx, ky):
+            return np.sin(mx * kx) * (np.sin(my * ky) * math.sqrt(4 / ((nx + 1) * (ny + 1))))
+    ch = {"kxdx": arrs["kt1"].reshape((nx, ny), order="F").copy(),
+          "kydx": arrs["kt2"].reshape((nx, ny), order="F").copy(), "fun_phi": fun_phi}
+    for s in ("L", "R"):
+        if s == "R" and not two_sided:
+            ch["R"] = {"N_prop": float("nan")}
+            continue
+        d = _side(c, arrs, iarrs, s, False)
+        d["kzdx"] = d.pop("kxdx").reshape((nx, ny), order="F")
+        ch[s] = d
+    if two_sided and complex(epsilon_R) == complex(epsilon_L):
+        ch["R"] = ch["L"]
+    return ch
+
+
+def build_laplacian_1d(n: int, BC):
+    """A = build_laplacian_1d(n, BC) as a scipy.sparse CSR matrix (build_laplacian_1d.m:1-77)."""
+    import scipy.sparse as sp
+    if not (isinstance(n, (int, np.integer)) and n >= 0):
+        raise ValueError("The number of sites n must be a non-negative integer scalar")
+    if _is_num(BC) and np.imag(BC) != 0:
+        warnings.warn(f"k*a = {np.real(BC):g} + 1i*{np.imag(BC):g} is a complex number.")
+        raise ValueError("complex Bloch phases are not supported by build_laplacian_1d")
+    bc, ka = _bc_1d(BC)
+    A = np.zeros(2 * n * n)
+    L.check(L.load_library().rgf_build_laplacian_1d(n, bc, ka, L.dptr(A)))
+    M = (A[0::2] + 1j * A[1::2]).reshape((n, n), order="F")
+    return sp.csr_matrix(M if np.any(M.imag) else M.real)
+
+
+def dense_inverse(A: np.ndarray, precision: str = "fp64", block_size: int = 0, device: int = -1) -> np.ndarray:
+    """In-place-style inverse through the slice-inverse kernel (kernel-level test entry)."""
+    lib = L.load_library()
+    M = np.asfortranarray(A, dtype=np.complex128).copy(order="F")
+    h = L.handle(device)
+    L.check(lib.rgf_dense_inverse(h, _PRECISIONS[precision], M.shape[0], M.ctypes.data_as(L._dp), block_size), h)
+    return M
+
+
+def dense_gemm(A: np.ndarray, B: np.ndarray, precision: str = "fp64", device: int = -1) -> np.ndarray:
+    lib = L.load_library()
+    A = np.asfortranarray(A, dtype=np.complex128)
+    B = np.asfortranarray(B, dtype=np.complex128)
+    Cm = np.zeros((A.shape[0], B.shape[1]), dtype=np.complex128, order="F")
+    h = L.handle(device)
+    L.check(lib.rgf_dense_gemm(h, _PRECISIONS[precision], A.shape[0], B.shape[1], A.shape[1],
+                               A.ctypes.data_as(L._dp), B.ctypes.data_as(L._dp), Cm.ctypes.data_as(L._dp)), h)
+    return Cm

@@ -1,0 +1,846 @@
+// C ABI of librgf and the slice-recursion drivers.
+//
+// Two drivers share the kernels of rgf_kernels.hip:
+//
+//  * carry (FP64): the reference's slice loop (cal_smatrix_RGF.m:215-280)
+//    with the propagators carried projected onto the requested channels:
+//      g_n = (A_n - g_{n-1})^-1,  X = g_n X,  Z += Y X,  Y = Y g_n
+//    where X = G_RL B, Y = C G_LR, Z = C G_LL B.  B and C are linear, so
+//    this equals the reference's G_RL/G_LR/G_LL products followed by its
+//    Fisher-Lee projection, at O(N^2 p) instead of O(N^3) per slice.  The
+//    final lead attach (I - G0_R g)\G0_R equals (A_lead - G0_R - g)^-1
+//    because G0_R = (A_lead - G0_R)^-1, so it is one more Dyson step.  The
+//    loop runs from the side with fewer requested channels, which reproduces
+//    the reference's choice of loop direction for all five schemes.
+//
+//  * refined (FP32 + FP64 refinement): the same Dyson recursion in complex
+//    FP32 on MFMA with every slice inverse g_n stored; the S-matrix columns
+//    are the block-tridiagonal solve H psi = b (slices 0..L+1, leads folded
+//    into the end blocks), corrected by FP64 residual iterative refinement
+//    psi += H~^-1 (b - H psi), where H~^-1 is the forward/backward sweep
+//    through the stored FP32 inverses and the residual uses the FP64 slice
+//    operators and FP64 lead Green's functions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "rgf_host.h"
+
+using namespace rgf;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct rgf_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, DevBuf> bufs;
+    int* d_status = nullptr;
+    double* d_scratch = nullptr;
+};
+
+static std::string g_last_error;
+static std::mutex g_err_mu;
+
+namespace {
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+T* dbuf(rgf_context* c, const std::string& name, size_t count) {
+    DevBuf& b = c->bufs[name];
+    const size_t need = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.bytes < need) {
+        if (b.p) {
+            RGF_HIP(hipStreamSynchronize(c->stream));
+            RGF_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        RGF_HIP(hipMalloc(&b.p, need));
+        b.bytes = need;
+    }
+    return (T*)b.p;
+}
+
+void release(rgf_context* c, const std::string& name) {
+    auto it = c->bufs.find(name);
+    if (it == c->bufs.end()) return;
+    if (it->second.p) {
+        RGF_HIP(hipStreamSynchronize(c->stream));
+        RGF_HIP(hipFree(it->second.p));
+    }
+    c->bufs.erase(it);
+}
+
+// planar device matrix rows x cols, row-major, named buffer
+template <typename T>
+CMat<T> dmat(rgf_context* c, const std::string& name, int64_t rows, int64_t cols) {
+    T* p = dbuf<T>(c, name, (size_t)2 * std::max<int64_t>(rows, 1) * std::max<int64_t>(cols, 1));
+    return CMat<T>(p, p + (size_t)std::max<int64_t>(rows, 1) * std::max<int64_t>(cols, 1), std::max<int64_t>(cols, 1));
+}
+
+// host complex row-major -> planar device
+void upload(rgf_context* c, const std::vector<cd>& h, int64_t rows, int64_t cols, CMat<double> d) {
+    std::vector<double> re((size_t)rows * cols), im((size_t)rows * cols);
+    for (int64_t i = 0; i < rows; ++i)
+        for (int64_t j = 0; j < cols; ++j) {
+            re[i * cols + j] = h[i * cols + j].real();
+            im[i * cols + j] = h[i * cols + j].imag();
+        }
+    for (int64_t i = 0; i < rows; ++i) {
+        RGF_HIP(hipMemcpyAsync(d.re + i * d.ld, re.data() + i * cols, cols * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+        RGF_HIP(hipMemcpyAsync(d.im + i * d.ld, im.data() + i * cols, cols * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    }
+    RGF_HIP(hipStreamSynchronize(c->stream));
+}
+
+std::vector<cd> download(rgf_context* c, CMat<double> d, int64_t rows, int64_t cols) {
+    std::vector<double> re((size_t)rows * cols), im((size_t)rows * cols);
+    for (int64_t i = 0; i < rows; ++i) {
+        RGF_HIP(hipMemcpyAsync(re.data() + i * cols, d.re + i * d.ld, cols * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+        RGF_HIP(hipMemcpyAsync(im.data() + i * cols, d.im + i * d.ld, cols * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    RGF_HIP(hipStreamSynchronize(c->stream));
+    std::vector<cd> h((size_t)rows * cols);
+    for (size_t q = 0; q < h.size(); ++q) h[q] = cd(re[q], im[q]);
+    return h;
+}
+
+void check_status(rgf_context* c) {
+    int st = 0;
+    RGF_HIP(hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RGF_HIP(hipStreamSynchronize(c->stream));
+    if (st) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+}
+
+void reset_status(rgf_context* c) { RGF_HIP(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream)); }
+
+// ---------------------------------------------------------------- problem on the device
+// Everything in sweep order: "L" is the lead the slice loop starts from.
+struct Work {
+    const Channels* ch = nullptr;
+    bool mirrored = false;
+    int64_t N = 0, L = 0;
+    cd k2, epsL, epsR;
+    const Side* sL = nullptr;
+    const Side* sR = nullptr;
+    std::vector<int32_t> inL, inR, outL, outR;  // 0-based, sweep-side
+    // device (FP64)
+    double* eps_re = nullptr;
+    double* eps_im = nullptr;  // N x L planar, slice n at n*N
+    DevCSR A0;
+    CMat<double> G0L, G0R, BL, BR, CL, CR;
+    // host results (FP64, row-major), projected Green's functions (no -2i)
+    std::vector<cd> TLL, TLR, TRL, TRR;
+    double flops = 0;
+    double t_setup = 0, t_slices = 0, t_refine = 0;
+    KernelTimer* timer = nullptr;
+    int refine_iters = 0;
+    double refine_residual = 0;
+};
+
+void upload_eps(rgf_context* c, const rgf_problem* p, Work& w) {
+    const int64_t N = w.N, L = w.L;
+    w.eps_re = dbuf<double>(c, "eps_re", (size_t)N * std::max<int64_t>(L, 1));
+    w.eps_im = p->eps_im ? dbuf<double>(c, "eps_im", (size_t)N * std::max<int64_t>(L, 1)) : nullptr;
+    const hipMemcpyKind kind = p->eps_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (L == 0) return;
+    if (!w.mirrored) {
+        RGF_HIP(hipMemcpyAsync(w.eps_re, p->eps_re, N * L * sizeof(double), kind, c->stream));
+        if (w.eps_im) RGF_HIP(hipMemcpyAsync(w.eps_im, p->eps_im, N * L * sizeof(double), kind, c->stream));
+    } else {
+        for (int64_t n = 0; n < L; ++n) {
+            RGF_HIP(hipMemcpyAsync(w.eps_re + n * N, p->eps_re + (L - 1 - n) * N, N * sizeof(double), kind, c->stream));
+            if (w.eps_im)
+                RGF_HIP(hipMemcpyAsync(w.eps_im + n * N, p->eps_im + (L - 1 - n) * N, N * sizeof(double), kind,
+                                       c->stream));
+        }
+    }
+}
+
+void upload_stencil(rgf_context* c, const Channels& ch, Work& w) {
+    HostCSR h = slice_stencil(ch);
+    DevCSR& d = w.A0;
+    d.n = h.n;
+    d.nnz = (int32_t)h.col.size();
+    d.rowptr = dbuf<int32_t>(c, "A0_rowptr", h.rowptr.size());
+    d.col = dbuf<int32_t>(c, "A0_col", h.col.size());
+    d.vre = dbuf<double>(c, "A0_vre", h.vre.size());
+    d.vim = dbuf<double>(c, "A0_vim", h.vim.size());
+    RGF_HIP(hipMemcpyAsync(d.rowptr, h.rowptr.data(), h.rowptr.size() * 4, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipMemcpyAsync(d.col, h.col.data(), h.col.size() * 4, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipMemcpyAsync(d.vre, h.vre.data(), h.vre.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipMemcpyAsync(d.vim, h.vim.data(), h.vim.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipStreamSynchronize(c->stream));
+}
+
+// G0 = phi diag(exp(i kl)) phi^H, FP64 (cal_smatrix_RGF.m:198-203)
+void lead_green(rgf_context* c, const Channels& ch, const Side& side, CMat<double> G0) {
+    const int64_t N = ch.N;
+    std::vector<double> f1(2 * ch.f1.size()), f2(2 * ch.f2.size()), g(2 * N);
+    for (size_t q = 0; q < ch.f1.size(); ++q) {
+        f1[2 * q] = ch.f1[q].real();
+        f1[2 * q + 1] = ch.f1[q].imag();
+    }
+    for (size_t q = 0; q < ch.f2.size(); ++q) {
+        f2[2 * q] = ch.f2[q].real();
+        f2[2 * q + 1] = ch.f2[q].imag();
+    }
+    for (int64_t a = 0; a < N; ++a) {
+        const cd e = std::exp(cd(0, 1) * side.kl[a]);
+        g[2 * a] = e.real();
+        g[2 * a + 1] = e.imag();
+    }
+    double* df1 = dbuf<double>(c, "f1", f1.size());
+    double* df2 = dbuf<double>(c, "f2", f2.size());
+    double* dg = dbuf<double>(c, "gdiag", g.size());
+    RGF_HIP(hipMemcpyAsync(df1, f1.data(), f1.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipMemcpyAsync(df2, f2.data(), f2.size() * 8, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipMemcpyAsync(dg, g.data(), g.size() * 8, hipMemcpyHostToDevice, c->stream));
+    CMat<double> P1 = dmat<double>(c, "tmpA", N, N), P2 = dmat<double>(c, "tmpB", N, N);
+    fill_modes(c->stream, ch.n1, ch.n2, df1, df2, dg, P1, P2);
+    cgemm<double>(c->stream, N, N, N, P1, P2, G0, cd(1, 0), 0);
+    RGF_HIP(hipStreamSynchronize(c->stream));
+}
+
+// B = phi(:, prop(in)) diag(sqrt_mu(in)) (N x n); C = diag(sqrt_mu(out)) phi(:, prop(out))^H
+// (cal_smatrix_RGF.m:189-195; sqrt_mu is not conjugated in C)
+std::vector<cd> make_B(const Channels& ch, const Side& s, const std::vector<int32_t>& idx) {
+    const int64_t N = ch.N, n = idx.size();
+    std::vector<cd> B((size_t)N * n);
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t a = s.ind_prop[idx[q]];
+        const cd mu = s.sqrt_mu[idx[q]];
+        for (int64_t i = 0; i < N; ++i) B[i * n + q] = mode_value(ch, i, a) * mu;
+    }
+    return B;
+}
+
+std::vector<cd> make_C(const Channels& ch, const Side& s, const std::vector<int32_t>& idx) {
+    const int64_t N = ch.N, n = idx.size();
+    std::vector<cd> C((size_t)n * N);
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t a = s.ind_prop[idx[q]];
+        const cd mu = s.sqrt_mu[idx[q]];
+        for (int64_t i = 0; i < N; ++i) C[q * N + i] = mu * std::conj(mode_value(ch, i, a));
+    }
+    return C;
+}
+
+void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Selection& sel, Work& w) {
+    w.ch = &ch;
+    w.N = ch.N;
+    w.L = ch.L_slices;
+    w.k2 = ch.k2;
+    w.mirrored = (sel.inR.size() + sel.outR.size()) < (sel.inL.size() + sel.outL.size());
+    if (!w.mirrored) {
+        w.sL = &ch.L;
+        w.sR = &ch.R;
+        w.epsL = ch.epsL;
+        w.epsR = ch.epsR;
+        w.inL = sel.inL;
+        w.inR = sel.inR;
+        w.outL = sel.outL;
+        w.outR = sel.outR;
+    } else {
+        w.sL = &ch.R;
+        w.sR = &ch.L;
+        w.epsL = ch.epsR;
+        w.epsR = ch.epsL;
+        w.inL = sel.inR;
+        w.inR = sel.inL;
+        w.outL = sel.outR;
+        w.outR = sel.outL;
+    }
+    const int64_t N = w.N;
+    upload_eps(c, p, w);
+    upload_stencil(c, ch, w);
+    w.G0L = dmat<double>(c, "G0L", N, N);
+    lead_green(c, ch, *w.sL, w.G0L);
+    if (ch.same_sides) {
+        w.G0R = w.G0L;
+    } else {
+        w.G0R = dmat<double>(c, "G0R", N, N);
+        lead_green(c, ch, *w.sR, w.G0R);
+    }
+    release(c, "tmpA");
+    release(c, "tmpB");
+    w.BL = dmat<double>(c, "BL", N, w.inL.size());
+    w.BR = dmat<double>(c, "BR", N, w.inR.size());
+    w.CL = dmat<double>(c, "CL", w.outL.size(), N);
+    w.CR = dmat<double>(c, "CR", w.outR.size(), N);
+    if (!w.inL.empty()) upload(c, make_B(ch, *w.sL, w.inL), N, w.inL.size(), w.BL);
+    if (!w.inR.empty()) upload(c, make_B(ch, *w.sR, w.inR), N, w.inR.size(), w.BR);
+    if (!w.outL.empty()) upload(c, make_C(ch, *w.sL, w.outL), w.outL.size(), N, w.CL);
+    if (!w.outR.empty()) upload(c, make_C(ch, *w.sR, w.outR), w.outR.size(), N, w.CR);
+}
+
+int default_block(int precision) { return precision == RGF_PRECISION_FP64 ? 64 : 128; }
+
+// ---------------------------------------------------------------- FP64 carry driver
+void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
+    const int64_t N = w.N, L = w.L;
+    const int64_t pi = w.inL.size(), po = w.outL.size();
+    const int b = o.block_size > 0 ? o.block_size : default_block(RGF_PRECISION_FP64);
+    hipStream_t s = c->stream;
+    CMat<double> G = dmat<double>(c, "G", N, N);
+    double* ws = dbuf<double>(c, "inv_ws", inverse_workspace_elems<double>(N, b));
+    CMat<double> X = dmat<double>(c, "X", N, pi), X2 = dmat<double>(c, "X2", N, pi);
+    CMat<double> Y = dmat<double>(c, "Y", po, N), Y2 = dmat<double>(c, "Y2", po, N);
+    CMat<double> Z = dmat<double>(c, "Z", po, pi);
+    copy_mat<double>(s, N, N, w.G0L, G);
+    if (pi) cgemm<double>(s, N, pi, N, w.G0L, w.BL, X, cd(1, 0), 0);
+    if (po) cgemm<double>(s, po, N, N, w.CL, w.G0L, Y, cd(1, 0), 0);
+    if (po && pi) cgemm<double>(s, po, pi, N, w.CL, X, Z, cd(1, 0), 0);
+    reset_status(c);
+    RGF_HIP(hipStreamSynchronize(s));
+    const double t0 = now();
+    for (int64_t n = 1; n <= L + 1; ++n) {
+        if (n <= L)
+            assemble_slice<double>(s, N, G, w.eps_re + (n - 1) * N, w.eps_im ? w.eps_im + (n - 1) * N : nullptr,
+                                   cd(0, 0), w.k2, nullptr, w.A0);
+        else
+            assemble_slice<double>(s, N, G, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
+        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer);
+        if (pi) {
+            cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0);
+            std::swap(X, X2);
+        }
+        if (po && pi) cgemm<double>(s, po, pi, N, Y, X, Z, cd(1, 0), 1);
+        if (po) {
+            cgemm<double>(s, po, N, N, Y, G, Y2, cd(1, 0), 0);
+            std::swap(Y, Y2);
+        }
+    }
+    RGF_HIP(hipStreamSynchronize(s));
+    w.t_slices = now() - t0;
+    const double Nd = (double)N;
+    w.flops = (double)(L + 1) * (8 * Nd * Nd * Nd + 8 * Nd * Nd * (pi + po) + 8.0 * po * Nd * pi);
+    check_status(c);
+    const int64_t qi = w.inR.size(), qo = w.outR.size();
+    if (po && pi) w.TLL = download(c, Z, po, pi);
+    if (qo && pi) {
+        CMat<double> T = dmat<double>(c, "T", qo, pi);
+        cgemm<double>(s, qo, pi, N, w.CR, X, T, cd(1, 0), 0);
+        w.TRL = download(c, T, qo, pi);
+    }
+    if (po && qi) {
+        CMat<double> T = dmat<double>(c, "T", po, qi);
+        cgemm<double>(s, po, qi, N, Y, w.BR, T, cd(1, 0), 0);
+        w.TLR = download(c, T, po, qi);
+    }
+    if (qo && qi) {
+        CMat<double> GB = dmat<double>(c, "GB", N, qi), T = dmat<double>(c, "T", qo, qi);
+        cgemm<double>(s, N, qi, N, G, w.BR, GB, cd(1, 0), 0);
+        cgemm<double>(s, qo, qi, N, w.CR, GB, T, cd(1, 0), 0);
+        w.TRR = download(c, T, qo, qi);
+    }
+}
+
+// ---------------------------------------------------------------- FP32 + refinement driver
+void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
+    const int64_t N = w.N, L = w.L, S = L + 2;
+    const int64_t mL = w.inL.size(), mR = w.inR.size(), m = mL + mR;
+    const int b = o.block_size > 0 ? o.block_size : default_block(RGF_PRECISION_FP32_REFINED);
+    const int use3m = o.complex_3m;
+    hipStream_t s = c->stream;
+    // factor store: S slices of N x N complex FP32
+    size_t freeb = 0, totalb = 0;
+    RGF_HIP(hipMemGetInfo(&freeb, &totalb));
+    const double need = (double)S * N * N * 8 + (double)S * N * m * (16 + 16 + 8) + (double)N * N * 8;
+    if (need > 0.92 * (double)freeb)
+        throw Error(RGF_ERR_OUT_OF_MEMORY, "mixed-precision factor store does not fit in device memory");
+    float* F = dbuf<float>(c, "factors", (size_t)S * N * N * 2);
+    auto Fn = [&](int64_t n) { return CMat<float>(F + (size_t)n * N * N * 2, F + (size_t)n * N * N * 2 + (size_t)N * N, N); };
+    float* ws = dbuf<float>(c, "inv_ws32", inverse_workspace_elems<float>(N, b));
+    CMat<float> G0R32 = dmat<float>(c, "G0R32", N, N);
+    convert<float, double>(s, N, N, w.G0L, Fn(0), 1.0);
+    convert<float, double>(s, N, N, w.G0R, G0R32, 1.0);
+    reset_status(c);
+    RGF_HIP(hipStreamSynchronize(s));
+    const double t0 = now();
+    for (int64_t n = 1; n <= L + 1; ++n) {
+        copy_mat<float>(s, N, N, Fn(n - 1), Fn(n));
+        if (n <= L)
+            assemble_slice<float>(s, N, Fn(n), w.eps_re + (n - 1) * N, w.eps_im ? w.eps_im + (n - 1) * N : nullptr,
+                                  cd(0, 0), w.k2, nullptr, w.A0);
+        else
+            assemble_slice<float>(s, N, Fn(n), nullptr, nullptr, w.epsR, w.k2, &G0R32, w.A0);
+        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer);
+    }
+    RGF_HIP(hipStreamSynchronize(s));
+    w.t_slices = now() - t0;
+    check_status(c);
+    const double Nd = (double)N;
+    w.flops = (double)(L + 1) * 8 * Nd * Nd * Nd * (use3m ? 0.75 : 1.0);
+    release(c, "G0R32");
+
+    // refinement: psi (FP64), r (FP64), d (FP32), slices of N x m
+    const double t1 = now();
+    const size_t SE = (size_t)N * m;
+    double* psi = dbuf<double>(c, "psi", 2 * S * SE);
+    double* r = dbuf<double>(c, "res", 2 * S * SE);
+    float* d = dbuf<float>(c, "delta", 2 * S * SE);
+    CMat<float> t = dmat<float>(c, "tmp32", N, m);
+    auto P = [&](double* base, int64_t n) { return CMat<double>(base + n * SE, base + S * SE + n * SE, m); };
+    auto D = [&](int64_t n) { return CMat<float>(d + n * SE, d + S * SE + n * SE, m); };
+    RGF_HIP(hipMemsetAsync(psi, 0, 2 * S * SE * sizeof(double), s));
+    double bmax = std::max(max_abs(s, N * mL, w.BL.re, w.BL.im, c->d_scratch),
+                           max_abs(s, N * mR, w.BR.re, w.BR.im, c->d_scratch));
+    if (bmax == 0) bmax = 1;
+    const int maxit = o.max_refine_iter > 0 ? o.max_refine_iter : 6;
+    const double tol = o.refine_tol > 0 ? o.refine_tol : 1e-13;
+    double rel = 1.0;
+    int it = 0;
+    for (;; ++it) {
+        // r = b - H psi
+        residual_stencil(s, N, L, m, psi, psi + S * SE, r, r + S * SE, w.eps_re, w.eps_im, w.epsL, w.epsR, w.k2, w.A0,
+                         w.BL.re, w.BL.im, mL, w.BR.re, w.BR.im, 0, 0);
+        cgemm<double>(s, N, m, N, w.G0L, P(psi, 0), P(r, 0), cd(1, 0), 1);
+        cgemm<double>(s, N, m, N, w.G0R, P(psi, L + 1), P(r, L + 1), cd(1, 0), 1);
+        const double rmax = max_abs(s, S * SE, r, r + S * SE, c->d_scratch);
+        rel = rmax / bmax;
+        if (!std::isfinite(rel)) throw Error(RGF_ERR_NUMERICAL, "non-finite residual in refinement");
+        if (rel <= tol || it >= maxit) break;
+        const double sc = 1.0 / rmax;
+        // forward sweep: d_n = g_n (r_n + d_{n-1})
+        for (int64_t n = 0; n < S; ++n) {
+            convert<float, double>(s, N, m, P(r, n), t, sc);
+            if (n > 0) add_mat<float>(s, N, m, t, D(n - 1), t);
+            cgemm<float>(s, N, m, N, Fn(n), t, D(n), cd(1, 0), 0, 0, 0, use3m);
+        }
+        // backward sweep: d_n += g_n d_{n+1}
+        for (int64_t n = S - 2; n >= 0; --n) cgemm<float>(s, N, m, N, Fn(n), D(n + 1), D(n), cd(1, 0), 1, 0, 0, use3m);
+        accumulate<float>(s, S * N, m, CMat<float>(d, d + S * SE, m), CMat<double>(psi, psi + S * SE, m), rmax);
+    }
+    w.refine_iters = it;
+    w.refine_residual = rel;
+    w.flops += (double)it * (2.0 * S - 1) * 8 * Nd * Nd * m * (use3m ? 0.75 : 1.0);
+    // S blocks from the boundary slices: C_L psi_0, C_R psi_{L+1}
+    const int64_t po = w.outL.size(), qo = w.outR.size();
+    if (po) {
+        CMat<double> T = dmat<double>(c, "T", po, m);
+        cgemm<double>(s, po, m, N, w.CL, P(psi, 0), T, cd(1, 0), 0);
+        auto h = download(c, T, po, m);
+        w.TLL.assign((size_t)po * mL, 0);
+        w.TLR.assign((size_t)po * mR, 0);
+        for (int64_t i = 0; i < po; ++i) {
+            for (int64_t j = 0; j < mL; ++j) w.TLL[i * mL + j] = h[i * m + j];
+            for (int64_t j = 0; j < mR; ++j) w.TLR[i * mR + j] = h[i * m + mL + j];
+        }
+    }
+    if (qo) {
+        CMat<double> T = dmat<double>(c, "T", qo, m);
+        cgemm<double>(s, qo, m, N, w.CR, P(psi, L + 1), T, cd(1, 0), 0);
+        auto h = download(c, T, qo, m);
+        w.TRL.assign((size_t)qo * mL, 0);
+        w.TRR.assign((size_t)qo * mR, 0);
+        for (int64_t i = 0; i < qo; ++i) {
+            for (int64_t j = 0; j < mL; ++j) w.TRL[i * mL + j] = h[i * m + j];
+            for (int64_t j = 0; j < mR; ++j) w.TRR[i * mR + j] = h[i * m + mL + j];
+        }
+    }
+    w.t_refine = now() - t1;
+    if (rel > tol) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "mixed-precision refinement stopped at relative residual %.3e (tolerance %.1e)", rel,
+                 tol);
+        throw Error(RGF_ERR_NOT_CONVERGED, buf);
+    }
+}
+
+// Fisher-Lee relation S = -2i C G B minus the Kronecker delta of matching
+// indices (cal_smatrix_RGF.m:279, 286-289), written column-major interleaved.
+void write_smatrix(const Work& w, const Selection& sel, double* S_out) {
+    const int64_t nOL = sel.outL.size(), nOR = sel.outR.size(), nIL = sel.inL.size(), nIR = sel.inR.size();
+    const int64_t nout = nOL + nOR, nin = nIL + nIR;
+    // block of (row side, col side) in user (original) sides
+    auto block = [&](bool rowR, bool colR, int64_t i, int64_t j) -> cd {
+        bool sr = rowR, sc = colR;
+        if (w.mirrored) {
+            sr = !sr;
+            sc = !sc;
+        }
+        const std::vector<cd>* T;
+        int64_t ncols;
+        if (!sr && !sc) {
+            T = &w.TLL;
+            ncols = w.inL.size();
+        } else if (!sr && sc) {
+            T = &w.TLR;
+            ncols = w.inR.size();
+        } else if (sr && !sc) {
+            T = &w.TRL;
+            ncols = w.inL.size();
+        } else {
+            T = &w.TRR;
+            ncols = w.inR.size();
+        }
+        if (T->empty()) return cd(0, 0);
+        return (*T)[i * ncols + j];
+    };
+    for (int64_t j = 0; j < nin; ++j) {
+        const bool colR = j >= nIL;
+        const int64_t jj = colR ? j - nIL : j;
+        for (int64_t i = 0; i < nout; ++i) {
+            const bool rowR = i >= nOL;
+            const int64_t ii = rowR ? i - nOL : i;
+            const cd v = cd(0, -2) * block(rowR, colR, ii, jj);
+            S_out[2 * (i + j * nout)] = v.real();
+            S_out[2 * (i + j * nout) + 1] = v.imag();
+        }
+    }
+    // delta via intersect: first occurrence of each common index value
+    auto sub_delta = [&](const std::vector<int32_t>& out, const std::vector<int32_t>& in, int64_t ro, int64_t co) {
+        std::map<int32_t, int64_t> first_in;
+        for (int64_t q = (int64_t)in.size() - 1; q >= 0; --q) first_in[in[q]] = q;
+        std::map<int32_t, int64_t> first_out;
+        for (int64_t q = (int64_t)out.size() - 1; q >= 0; --q) first_out[out[q]] = q;
+        for (auto& e : first_out) {
+            auto it = first_in.find(e.first);
+            if (it == first_in.end()) continue;
+            S_out[2 * ((ro + e.second) + (co + it->second) * nout)] -= 1.0;
+        }
+    };
+    sub_delta(sel.outL, sel.inL, 0, 0);
+    sub_delta(sel.outR, sel.inR, nOL, nIL);
+}
+
+void fill_info(rgf_info* info, const Channels& ch, const Selection& sel) {
+    if (!info) return;
+    memset(info, 0, sizeof *info);
+    info->n_transverse = ch.N;
+    info->n_slices = ch.L_slices;
+    info->n_prop_L = ch.L.n_prop;
+    info->n_prop_R = ch.R.n_prop;
+    info->n_out = (int32_t)(sel.outL.size() + sel.outR.size());
+    info->n_in = (int32_t)(sel.inL.size() + sel.inR.size());
+}
+
+void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt, double* S_out,
+               rgf_info* info) {
+    const double t0 = now();
+    rgf_options o;
+    rgf_default_options(&o);
+    if (opt) o = *opt;
+    if (o.precision != RGF_PRECISION_FP64 && o.precision != RGF_PRECISION_FP32_REFINED)
+        throw Error(RGF_ERR_INVALID_ARGUMENT, "unknown precision");
+    Channels ch = setup_channels(p);
+    Selection sel = make_selection(s, ch);
+    fill_info(info, ch, sel);
+    const int64_t nout = sel.outL.size() + sel.outR.size(), nin = sel.inL.size() + sel.inR.size();
+    if (nout == 0 || nin == 0) return;  // cal_smatrix_RGF.m:181-184
+    if (!S_out) throw Error(RGF_ERR_INVALID_ARGUMENT, "S_out is NULL");
+    RGF_HIP(hipSetDevice(c->device));
+    Work w;
+    std::unique_ptr<KernelTimer> timer;
+    if (o.profile) {
+        timer.reset(new KernelTimer);
+        w.timer = timer.get();
+    }
+    prepare(c, p, ch, sel, w);
+    w.t_setup = now() - t0;
+    if (o.precision == RGF_PRECISION_FP64)
+        run_carry(c, w, o);
+    else
+        run_refined(c, w, o);
+    write_smatrix(w, sel, S_out);
+    if (info) {
+        info->swept_right_to_left = w.mirrored;
+        info->refine_iters = w.refine_iters;
+        info->refine_residual = w.refine_residual;
+        info->seconds_setup = w.t_setup;
+        info->seconds_slices = w.t_slices;
+        info->seconds_refine = w.t_refine;
+        info->flops_slices = w.flops;
+        if (timer) {
+            info->kernel_seconds = timer->seconds();
+            info->kernel_flops = timer->flops;
+            info->kernel_launches = timer->launches;
+        }
+        info->seconds_total = now() - t0;
+    }
+}
+
+rgf_status guard(rgf_context* c, const std::function<void()>& f) {
+    try {
+        f();
+        if (c) c->err.clear();
+        return RGF_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.what();
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host out of memory";
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_last_error = "host out of memory";
+        return RGF_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_last_error = e.what();
+        return RGF_ERR_INVALID_ARGUMENT;
+    }
+}
+
+// interleaved column-major (host) <-> planar row-major (device)
+void upload_colmajor(rgf_context* c, const double* A, int64_t rows, int64_t cols, CMat<double> d) {
+    std::vector<cd> h((size_t)rows * cols);
+    for (int64_t i = 0; i < rows; ++i)
+        for (int64_t j = 0; j < cols; ++j) h[i * cols + j] = cd(A[2 * (i + j * rows)], A[2 * (i + j * rows) + 1]);
+    upload(c, h, rows, cols, d);
+}
+
+void download_colmajor(rgf_context* c, CMat<double> d, int64_t rows, int64_t cols, double* A) {
+    auto h = download(c, d, rows, cols);
+    for (int64_t i = 0; i < rows; ++i)
+        for (int64_t j = 0; j < cols; ++j) {
+            A[2 * (i + j * rows)] = h[i * cols + j].real();
+            A[2 * (i + j * rows) + 1] = h[i * cols + j].imag();
+        }
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int32_t rgf_abi_version(void) { return RGF_ABI_VERSION; }
+
+const char* rgf_status_string(rgf_status s) {
+    switch (s) {
+        case RGF_OK: return "ok";
+        case RGF_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case RGF_ERR_BOUNDARY_CONDITION: return "invalid boundary condition";
+        case RGF_ERR_CHANNEL_INDEX: return "channel index out of range";
+        case RGF_ERR_OUT_OF_MEMORY: return "out of memory";
+        case RGF_ERR_DEVICE: return "device error";
+        case RGF_ERR_NUMERICAL: return "numerical failure";
+        case RGF_ERR_NOT_CONVERGED: return "refinement not converged";
+        case RGF_ERR_BUFFER_TOO_SMALL: return "buffer too small";
+    }
+    return "unknown status";
+}
+
+void rgf_default_options(rgf_options* o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->precision = RGF_PRECISION_FP64;
+    o->max_refine_iter = 6;
+    o->refine_tol = 1e-13;
+    o->block_size = 0;
+    o->complex_3m = 0;
+}
+
+rgf_status rgf_create(rgf_handle* h, int32_t device) {
+    return guard(nullptr, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "handle pointer is NULL");
+        *h = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw Error(RGF_ERR_DEVICE, "no HIP device");
+        std::unique_ptr<rgf_context> c(new rgf_context);
+        if (device < 0) RGF_HIP(hipGetDevice(&device));
+        if (device >= count) throw Error(RGF_ERR_DEVICE, "device index out of range");
+        c->device = device;
+        RGF_HIP(hipSetDevice(device));
+        RGF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        c->stream = c->own;
+        RGF_HIP(hipMalloc(&c->d_status, 256));
+        RGF_HIP(hipMalloc(&c->d_scratch, 256));
+        *h = c.release();
+    });
+}
+
+rgf_status rgf_destroy(rgf_handle h) {
+    if (!h) return RGF_OK;
+    rgf_status st = guard(nullptr, [&] {
+        RGF_HIP(hipSetDevice(h->device));
+        RGF_HIP(hipStreamSynchronize(h->stream));
+        for (auto& e : h->bufs)
+            if (e.second.p) RGF_HIP(hipFree(e.second.p));
+        h->bufs.clear();
+        if (h->d_status) RGF_HIP(hipFree(h->d_status));
+        if (h->d_scratch) RGF_HIP(hipFree(h->d_scratch));
+        if (h->own) RGF_HIP(hipStreamDestroy(h->own));
+    });
+    delete h;
+    return st;
+}
+
+const char* rgf_last_error(rgf_handle h) {
+    if (h) return h->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_last_error.c_str();
+}
+
+rgf_status rgf_set_stream(rgf_handle h, void* stream) {
+    return guard(h, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
+        h->stream = stream ? (hipStream_t)stream : h->own;
+    });
+}
+
+rgf_status rgf_trim(rgf_handle h) {
+    return guard(h, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
+        RGF_HIP(hipSetDevice(h->device));
+        RGF_HIP(hipStreamSynchronize(h->stream));
+        for (auto& e : h->bufs)
+            if (e.second.p) RGF_HIP(hipFree(e.second.p));
+        h->bufs.clear();
+    });
+}
+
+rgf_status rgf_query(const rgf_problem* p, const rgf_selection* s, int32_t* nL, int32_t* nR, int32_t* nout,
+                     int32_t* nin) {
+    return guard(nullptr, [&] {
+        Channels ch = setup_channels(p);
+        if (nL) *nL = ch.L.n_prop;
+        if (nR) *nR = ch.R.n_prop;
+        if (s) {
+            Selection sel = make_selection(s, ch);
+            if (nout) *nout = (int32_t)(sel.outL.size() + sel.outR.size());
+            if (nin) *nin = (int32_t)(sel.inL.size() + sel.inR.size());
+        }
+    });
+}
+
+rgf_status rgf_setup_channels(const rgf_problem* p, rgf_channels* out) {
+    return guard(nullptr, [&] {
+        if (!out) throw Error(RGF_ERR_INVALID_ARGUMENT, "channels output is NULL");
+        Channels ch = setup_channels(p);
+        const int64_t N = ch.N;
+        for (int64_t a = 0; a < N; ++a) {
+            if (out->kt1) out->kt1[a] = ch.kt1[a];
+            if (out->kt2 && ch.ndim == 3) out->kt2[a] = ch.kt2[a];
+            if (out->kl_L) {
+                out->kl_L[2 * a] = ch.L.kl[a].real();
+                out->kl_L[2 * a + 1] = ch.L.kl[a].imag();
+            }
+            if (out->kl_R) {
+                out->kl_R[2 * a] = ch.R.kl[a].real();
+                out->kl_R[2 * a + 1] = ch.R.kl[a].imag();
+            }
+        }
+        const Side* sides[2] = {&ch.L, &ch.R};
+        int32_t* ip[2] = {out->ind_prop_L, out->ind_prop_R};
+        double* mu[2] = {out->sqrt_mu_L, out->sqrt_mu_R};
+        int32_t* cj[2] = {out->ind_prop_conj_L, out->ind_prop_conj_R};
+        for (int k = 0; k < 2; ++k)
+            for (int32_t q = 0; q < sides[k]->n_prop; ++q) {
+                if (ip[k]) ip[k][q] = sides[k]->ind_prop[q] + 1;
+                if (mu[k]) {
+                    mu[k][2 * q] = sides[k]->sqrt_mu[q].real();
+                    mu[k][2 * q + 1] = sides[k]->sqrt_mu[q].imag();
+                }
+                if (cj[k] && sides[k]->has_conj) cj[k][q] = sides[k]->conj[q] + 1;
+            }
+        out->n_prop_L = ch.L.n_prop;
+        out->n_prop_R = ch.R.n_prop;
+        out->has_conj = ch.L.has_conj ? 1 : 0;
+    });
+}
+
+rgf_status rgf_build_laplacian_1d(int64_t n, int32_t bc, double ka, double* A) {
+    return guard(nullptr, [&] {
+        if (n < 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "The number of sites n must be a non-negative integer scalar");
+        if (bc < RGF_BC_BLOCH || bc > RGF_BC_NEUMANN_DIRICHLET)
+            throw Error(RGF_ERR_BOUNDARY_CONDITION, "Boundary condition BC is not an eligible option");
+        if (!A) throw Error(RGF_ERR_INVALID_ARGUMENT, "output is NULL");
+        memset(A, 0, sizeof(double) * 2 * n * n);
+        for (auto& e : laplacian_1d(n, bc, ka)) {
+            A[2 * (e.first.first + e.first.second * n)] = e.second.real();
+            A[2 * (e.first.first + e.first.second * n) + 1] = e.second.imag();
+        }
+    });
+}
+
+rgf_status rgf_solve(rgf_handle h, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt,
+                     double* S_out, rgf_info* info) {
+    return guard(h, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
+        solve_one(h, p, s, opt, S_out, info);
+    });
+}
+
+rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, const rgf_selection* s,
+                             const rgf_options* opt, double* const* S_out, rgf_info* info) {
+    return guard(h, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
+        if (count < 0 || (count > 0 && (!p || !s || !S_out))) throw Error(RGF_ERR_INVALID_ARGUMENT, "bad batch");
+        for (int32_t q = 0; q < count; ++q) solve_one(h, &p[q], &s[q], opt, S_out[q], info ? &info[q] : nullptr);
+    });
+}
+
+rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem*, const rgf_selection*, const rgf_options*,
+                                     int32_t, double*, double*, double*, rgf_info*) {
+    return guard(h, [&] { throw Error(RGF_ERR_INVALID_ARGUMENT, "transmission gradient not available"); });
+}
+
+rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double* A, int32_t block_size) {
+    return guard(h, [&] {
+        if (!h || !A || n <= 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "bad dense inverse arguments");
+        RGF_HIP(hipSetDevice(h->device));
+        CMat<double> M = dmat<double>(h, "dense_A", n, n);
+        upload_colmajor(h, A, n, n, M);
+        reset_status(h);
+        if (precision == RGF_PRECISION_FP64) {
+            const int b = block_size > 0 ? block_size : default_block(precision);
+            double* ws = dbuf<double>(h, "inv_ws", inverse_workspace_elems<double>(n, b));
+            invert_inplace<double>(h->stream, n, M, b, ws, h->d_status);
+        } else {
+            const int b = block_size > 0 ? block_size : default_block(precision);
+            CMat<float> M32 = dmat<float>(h, "dense_A32", n, n);
+            convert<float, double>(h->stream, n, n, M, M32, 1.0);
+            float* ws = dbuf<float>(h, "inv_ws32", inverse_workspace_elems<float>(n, b));
+            invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status);
+            convert<double, float>(h->stream, n, n, M32, M, 1.0);
+        }
+        check_status(h);
+        download_colmajor(h, M, n, n, A);
+    });
+}
+
+rgf_status rgf_dense_gemm(rgf_handle h, int32_t precision, int64_t m, int64_t n, int64_t k, const double* A,
+                          const double* B, double* C) {
+    return guard(h, [&] {
+        if (!h || !A || !B || !C || m <= 0 || n <= 0 || k <= 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "bad gemm");
+        RGF_HIP(hipSetDevice(h->device));
+        CMat<double> dA = dmat<double>(h, "gA", m, k), dB = dmat<double>(h, "gB", k, n), dC = dmat<double>(h, "gC", m, n);
+        upload_colmajor(h, A, m, k, dA);
+        upload_colmajor(h, B, k, n, dB);
+        if (precision == RGF_PRECISION_FP64) {
+            cgemm<double>(h->stream, m, n, k, dA, dB, dC, cd(1, 0), 0);
+        } else {
+            CMat<float> a = dmat<float>(h, "gA32", m, k), b = dmat<float>(h, "gB32", k, n), c = dmat<float>(h, "gC32", m, n);
+            convert<float, double>(h->stream, m, k, dA, a, 1.0);
+            convert<float, double>(h->stream, k, n, dB, b, 1.0);
+            cgemm<float>(h->stream, m, n, k, a, b, c, cd(1, 0), 0);
+            convert<double, float>(h->stream, m, n, c, dC, 1.0);
+        }
+        download_colmajor(h, dC, m, n, C);
+    });
+}
+
+}  // extern "C"

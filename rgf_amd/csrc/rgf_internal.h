@@ -1,0 +1,123 @@
+// Internal declarations shared by the host driver and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rgf.h"
+
+namespace rgf {
+
+using cd = std::complex<double>;
+
+struct Error : std::runtime_error {
+    rgf_status status;
+    Error(rgf_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define RGF_HIP(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            throw ::rgf::Error(_e == hipErrorOutOfMemory ? RGF_ERR_OUT_OF_MEMORY : RGF_ERR_DEVICE, \
+                               std::string(#expr) + ": " + hipGetErrorString(_e));          \
+    } while (0)
+
+// Planar complex matrix view on the device: element (i, j) of a row-major
+// matrix is re[i*ld + j] + 1i*im[i*ld + j].
+template <typename T>
+struct CMat {
+    T* re = nullptr;
+    T* im = nullptr;
+    int64_t ld = 0;
+    CMat() = default;
+    CMat(T* r, T* i, int64_t l) : re(r), im(i), ld(l) {}
+    CMat offset(int64_t i, int64_t j) const { return CMat(re + i * ld + j, im + i * ld + j, ld); }
+};
+
+// Sparse slice operator A0 (CSR) on the device, FP64.
+struct DevCSR {
+    int32_t* rowptr = nullptr;
+    int32_t* col = nullptr;
+    double* vre = nullptr;
+    double* vim = nullptr;
+    int32_t n = 0;
+    int32_t nnz = 0;
+};
+
+// Event timing of one kernel class on the stream it is launched on.
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    double flops = 0;
+    int64_t launches = 0;
+    void begin(hipStream_t s);
+    void end(hipStream_t s, double kernel_flops);
+    double seconds();  // synchronizes on the recorded events
+    ~KernelTimer();
+};
+
+// ---- kernel launchers (rgf_kernels.hip) -------------------------------------
+// C = alpha * A(MxK) * B(KxN) + beta * C, beta in {0,1}.  Tiles overlapping
+// rows or columns [skip_lo, skip_hi) are left untouched (block Gauss-Jordan).
+template <typename T>
+void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C,
+           cd alpha, int beta, int64_t skip_lo = 0, int64_t skip_hi = 0, int use3m = 0);
+
+// In-place inverse of the N x N matrix A by blocked Gauss-Jordan elimination
+// with partial pivoting inside each diagonal block.  ws must hold
+// inverse_workspace_elems<T>(N, b) elements of T.  *dev_status != 0 after the
+// call signals a zero or non-finite pivot.
+template <typename T>
+size_t inverse_workspace_elems(int64_t N, int b);
+template <typename T>
+void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_status, int use3m = 0,
+                    KernelTimer* timer = nullptr);
+
+// M <- A0 - diag(k2*eps) - M - Gextra  (Gextra may be null).  eps is a
+// planar FP64 vector (re, im; im may be null) or, when eps_re is null, the
+// constant eps_const.
+template <typename T>
+void assemble_slice(hipStream_t s, int64_t N, CMat<T> M, const double* eps_re, const double* eps_im,
+                    cd eps_const, cd k2, const CMat<T>* Gextra, const DevCSR& A0);
+
+// Dense helpers.
+template <typename T>
+void fill_zero(hipStream_t s, int64_t rows, int64_t cols, CMat<T> A);
+template <typename T>
+void set_identity(hipStream_t s, int64_t N, CMat<T> A);
+template <typename Td, typename Ts>
+void convert(hipStream_t s, int64_t rows, int64_t cols, CMat<Ts> src, CMat<Td> dst, double scale = 1.0);
+template <typename T>
+void copy_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> src, CMat<T> dst);
+// dst(FP64) += scale * src(T)
+template <typename T>
+void accumulate(hipStream_t s, int64_t rows, int64_t cols, CMat<T> src, CMat<double> dst, double scale);
+// dst = a + b (same type)
+template <typename T>
+void add_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> a, CMat<T> b, CMat<T> dst);
+
+// Mode-matrix fill: P1[i][a] = phi(i,a) * g[a], P2[a][i] = conj(phi(i,a)),
+// phi((i1,i2),(a1,a2)) = f1[i1][a1] f2[i2][a2]; f1 is n1 x n1, f2 n2 x n2
+// (row-major interleaved FP64), g length N interleaved (may be null -> 1).
+void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2,
+                const double* g, CMat<double> P1, CMat<double> P2);
+
+// Residual of the block-tridiagonal system for slices 0..L+1 (FP64):
+// r_n = b_n - (A0 - k2 diag(eps_n)) psi_n + psi_{n-1} + psi_{n+1}.  The
+// lead self-energy terms at slices 0 and L+1 are added by the caller.
+void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const double* psi_re,
+                      const double* psi_im, double* r_re, double* r_im, const double* eps_re,
+                      const double* eps_im, cd epsL, cd epsR, cd k2, const DevCSR& A0,
+                      const double* b0_re, const double* b0_im, int64_t mL, const double* b1_re,
+                      const double* b1_im, int64_t slice_stride, int reversed);
+
+// max |x| over a planar FP64 array (result to host, synchronizes the stream).
+double max_abs(hipStream_t s, int64_t count, const double* re, const double* im, double* dev_scratch);
+
+}  // namespace rgf

@@ -1,0 +1,767 @@
+// gfx950 kernels of the RGF slice recursion.
+//
+//  K1 assemble_slice   M = A0 - k^2 diag(eps_n) - G_{n-1}: one dense pass
+//                      (HBM-bound) plus a CSR pass for the stencil A0.
+//  K2 invert_inplace   blocked complex Gauss-Jordan inverse; the rank-b
+//                      trailing updates run on MFMA through cgemm.
+//  K3 cgemm            complex GEMM on MFMA (v_mfma_f64_16x16x4 /
+//                      v_mfma_f32_16x16x4), planar re/im operands, 4M or 3M.
+//  K4 fill_modes       transverse mode matrices for the lead Green's function.
+//  K5 (projection)     Fisher-Lee products are cgemm calls from the driver.
+//  residual_stencil    FP64 residual of the block-tridiagonal system used by
+//                      the mixed-precision refinement.
+//
+// Complex matrices are planar (separate re and im planes) and row-major, so
+// every MFMA operand is a plain real fragment and the four (or three) real
+// products of a complex product are independent MFMA chains.
+
+#include "rgf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace rgf {
+
+// ------------------------------------------------------------------ MFMA wrappers
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // accumulator element r of lane l holds row (l/16) + 4r
+    static __device__ __forceinline__ int row(int l4, int r) { return l4 + 4 * r; }
+};
+
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator element r of lane l holds row 4(l/16) + r
+    static __device__ __forceinline__ int row(int l4, int r) { return 4 * l4 + r; }
+};
+
+// ------------------------------------------------------------------ K3: complex GEMM
+// Work-group tile BM x BN, K chunk BK, WM x WN wavefronts; each wavefront owns
+// a (BM/WM) x (BN/WN) sub-tile made of 16x16 MFMA blocks.  MFMA operand
+// layout (16x16x4): lane l supplies A[l%16][l/16] and B[l/16][l%16]; the
+// accumulator element r of lane l is D[Mfma<T>::row(l/16, r)][l%16], which is
+// 4(l/16)+r for f32 and (l/16)+4r for f64 (measured on gfx950).
+//
+// USE3M: Re = ArBr - AiBi, Im = (Ar+Ai)(Br+Bi) - ArBr - AiBi (three MFMA
+// chains instead of four; the operand sums are formed once per LDS tile).
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool USE3M>
+__global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
+    const T* __restrict__ Are, const T* __restrict__ Aim, int64_t lda, const T* __restrict__ Bre,
+    const T* __restrict__ Bim, int64_t ldb, T* Cre, T* Cim, int64_t ldc, int M, int N, int K,
+    double alpha_re, double alpha_im, int beta, int skip_lo, int skip_hi) {
+    using V = typename Mfma<T>::v4;
+    constexpr int NT = WM * WN * 64;
+    constexpr int TM = BM / WM, TN = BN / WN;
+    constexpr int FM = TM / 16, FN = TN / 16;
+    constexpr int NP = USE3M ? 3 : 2;  // LDS planes: re, im (, re+im)
+    __shared__ T sA[NP][BM][BK + 1];
+    __shared__ T sB[NP][BK][BN + 1];
+
+    const int tile_n = blockIdx.x * BN;
+    const int tile_m = blockIdx.y * BM;
+    // tiles lying entirely inside the skipped band exit; partially covered
+    // tiles are masked element-wise in the epilogue
+    if (skip_hi > skip_lo) {
+        const bool rs = tile_m >= skip_lo && tile_m + BM <= skip_hi;
+        const bool cs = tile_n >= skip_lo && tile_n + BN <= skip_hi;
+        if (rs || cs) return;
+    }
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int l16 = lane & 15, l4 = lane >> 4;
+
+    V acc0[FM][FN], acc1[FM][FN], acc2[FM][FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+            acc0[a][b] = V{0, 0, 0, 0};
+            acc1[a][b] = V{0, 0, 0, 0};
+            acc2[a][b] = V{0, 0, 0, 0};
+        }
+
+    constexpr int AE = BM * BK, BE = BK * BN;
+    constexpr int AQ = (AE + NT - 1) / NT, BQ = (BE + NT - 1) / NT;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int e = tid + q * NT;
+            if (e < AE) {
+                const int i = e / BK, k = e % BK;
+                const int gi = tile_m + i, gk = k0 + k;
+                T vr = 0, vi = 0;
+                if (gi < M && gk < K) {
+                    vr = Are[(int64_t)gi * lda + gk];
+                    vi = Aim[(int64_t)gi * lda + gk];
+                }
+                sA[0][i][k] = vr;
+                sA[1][i][k] = vi;
+                if (USE3M) sA[NP - 1][i][k] = vr + vi;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int e = tid + q * NT;
+            if (e < BE) {
+                const int k = e / BN, j = e % BN;
+                const int gk = k0 + k, gj = tile_n + j;
+                T vr = 0, vi = 0;
+                if (gk < K && gj < N) {
+                    vr = Bre[(int64_t)gk * ldb + gj];
+                    vi = Bim[(int64_t)gk * ldb + gj];
+                }
+                sB[0][k][j] = vr;
+                sB[1][k][j] = vi;
+                if (USE3M) sB[NP - 1][k][j] = vr + vi;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+            for (int a = 0; a < FM; ++a) {
+                const int r = wm * TM + a * 16 + l16;
+                ar[a] = sA[0][r][kk + l4];
+                ai[a] = sA[1][r][kk + l4];
+            }
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+                const int c = wn * TN + b * 16 + l16;
+                br[b] = sB[0][kk + l4][c];
+                bi[b] = sB[1][kk + l4][c];
+            }
+            if (USE3M) {
+                T as[FM], bs[FN];
+#pragma unroll
+                for (int a = 0; a < FM; ++a) as[a] = sA[NP - 1][wm * TM + a * 16 + l16][kk + l4];
+#pragma unroll
+                for (int b = 0; b < FN; ++b) bs[b] = sB[NP - 1][kk + l4][wn * TN + b * 16 + l16];
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) {
+                        acc0[a][b] = Mfma<T>::mma(ar[a], br[b], acc0[a][b]);  // ArBr
+                        acc1[a][b] = Mfma<T>::mma(ai[a], bi[b], acc1[a][b]);  // AiBi
+                        acc2[a][b] = Mfma<T>::mma(as[a], bs[b], acc2[a][b]);  // (Ar+Ai)(Br+Bi)
+                    }
+            } else {
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) {
+                        acc0[a][b] = Mfma<T>::mma(ar[a], br[b], acc0[a][b]);
+                        acc0[a][b] = Mfma<T>::mma(-ai[a], bi[b], acc0[a][b]);
+                        acc1[a][b] = Mfma<T>::mma(ar[a], bi[b], acc1[a][b]);
+                        acc1[a][b] = Mfma<T>::mma(ai[a], br[b], acc1[a][b]);
+                    }
+            }
+        }
+        __syncthreads();
+    }
+
+    // epilogue: C = alpha * acc (+ C)
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(l4, r);
+                const int gj = tile_n + wn * TN + b * 16 + l16;
+                const bool masked = (gi >= skip_lo && gi < skip_hi) || (gj >= skip_lo && gj < skip_hi);
+                if (gi < M && gj < N && !masked) {
+                    double pr, pi;
+                    if (USE3M) {
+                        pr = (double)acc0[a][b][r] - (double)acc1[a][b][r];
+                        pi = (double)acc2[a][b][r] - (double)acc0[a][b][r] - (double)acc1[a][b][r];
+                    } else {
+                        pr = acc0[a][b][r];
+                        pi = acc1[a][b][r];
+                    }
+                    double vr = alpha_re * pr - alpha_im * pi;
+                    double vi = alpha_re * pi + alpha_im * pr;
+                    const int64_t o = (int64_t)gi * ldc + gj;
+                    if (beta) {
+                        vr += (double)Cre[o];
+                        vi += (double)Cim[o];
+                    }
+                    Cre[o] = (T)vr;
+                    Cim[o] = (T)vi;
+                }
+            }
+}
+
+template <typename T>
+struct GemmCfg;
+// FP64: 64x64 tiles, 4 waves (32x32 per wave); 128x128, 8 waves (64x32).
+// FP32: same shapes with a deeper K chunk.
+template <>
+struct GemmCfg<double> {
+    static constexpr int BK_SMALL = 16, BK_BIG = 16;
+};
+template <>
+struct GemmCfg<float> {
+    static constexpr int BK_SMALL = 32, BK_BIG = 32;
+};
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C,
+                         cd alpha, int beta, int64_t skip_lo, int64_t skip_hi, int use3m) {
+    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
+    dim3 block(WM * WN * 64);
+    if (use3m)
+        hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, true>), grid, block, 0, s, A.re, A.im, A.ld,
+                           B.re, B.im, B.ld, C.re, C.im, C.ld, (int)M, (int)N, (int)K, alpha.real(),
+                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi);
+    else
+        hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, false>), grid, block, 0, s, A.re, A.im, A.ld,
+                           B.re, B.im, B.ld, C.re, C.im, C.ld, (int)M, (int)N, (int)K, alpha.real(),
+                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi);
+}
+
+template <typename T>
+void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
+           int beta, int64_t skip_lo, int64_t skip_hi, int use3m) {
+    if (M <= 0 || N <= 0) return;
+    if (K <= 0) {
+        if (!beta) fill_zero<T>(s, M, N, C);
+        return;
+    }
+    if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX) throw Error(RGF_ERR_INVALID_ARGUMENT, "GEMM too large");
+    const int64_t tiles_big = ((M + 127) / 128) * ((N + 127) / 128);
+    if (tiles_big >= 1024)
+        launch_cgemm<T, 128, 128, GemmCfg<T>::BK_BIG, 2, 4>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
+    else
+        launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
+    RGF_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ K2: diagonal-block inverse
+// One work group inverts the nb x nb diagonal block D = A[p0:p0+nb, p0:p0+nb]
+// in LDS by Gauss-Jordan elimination with partial (row) pivoting inside the
+// block, and writes D^-1 back to A and to Dout (ld = ldd).  The column
+// interchanges that undo the row pivoting are folded into the write-back.
+template <typename T, int BMAX, int NT>
+__global__ __launch_bounds__(NT) void gj_diag_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb, T* Dre,
+                                                     T* Dim, int64_t ldd, int* status) {
+    __shared__ T sr[BMAX][BMAX + 1];
+    __shared__ T si[BMAX][BMAX + 1];
+    __shared__ T rowr[BMAX], rowi[BMAX], colr[BMAX], coli[BMAX];
+    __shared__ int pivs[BMAX];
+    __shared__ int colsrc[BMAX];
+    __shared__ int piv_s;
+    __shared__ T pvr_s, pvi_s;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nb * nb; e += NT) {
+        const int i = e / nb, j = e % nb;
+        sr[i][j] = Are[(int64_t)(p0 + i) * lda + p0 + j];
+        si[i][j] = Aim[(int64_t)(p0 + i) * lda + p0 + j];
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int k = 0; k < nb; ++k) {
+        if (tid < 64) {  // pivot search by wavefront 0
+            T best = -1;
+            int bi = k;
+            for (int r = k + tid; r < nb; r += 64) {
+                const T m = sr[r][k] * sr[r][k] + si[r][k] * si[r][k];
+                if (m > best) {
+                    best = m;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const T ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (tid == 0) {
+                piv_s = bi;
+                pvr_s = sr[bi][k];
+                pvi_s = si[bi][k];
+                pivs[k] = bi;
+            }
+        }
+        __syncthreads();
+        const int piv = piv_s;
+        const T pr = pvr_s, pim = pvi_s;
+        const T den = pr * pr + pim * pim;
+        const T ivr = pr / den, ivi = -pim / den;  // 1 / pivot
+        if (!(den > 0) || !isfinite(den)) bad = 1;
+        // new row k = old row piv scaled; old row k moves to row piv; save column k
+        for (int j = tid; j < nb; j += NT) {
+            const T xr = sr[piv][j], xi = si[piv][j];
+            if (j == k) {
+                rowr[j] = ivr;
+                rowi[j] = ivi;
+            } else {
+                rowr[j] = xr * ivr - xi * ivi;
+                rowi[j] = xr * ivi + xi * ivr;
+            }
+            if (piv != k) {
+                sr[piv][j] = sr[k][j];
+                si[piv][j] = si[k][j];
+            }
+        }
+        for (int r = tid; r < nb; r += NT) {
+            const T cr = (r == piv) ? sr[k][k] : sr[r][k];
+            const T ci = (r == piv) ? si[k][k] : si[r][k];
+            colr[r] = cr;
+            coli[r] = ci;
+        }
+        __syncthreads();
+        for (int e = tid; e < nb * nb; e += NT) {
+            const int r = e / nb, j = e % nb;
+            if (r == k) {
+                sr[r][j] = rowr[j];
+                si[r][j] = rowi[j];
+            } else if (j == k) {
+                const T cr = colr[r], ci = coli[r];
+                sr[r][j] = -(cr * ivr - ci * ivi);
+                si[r][j] = -(cr * ivi + ci * ivr);
+            } else {
+                const T cr = colr[r], ci = coli[r];
+                const T xr = rowr[j], xi = rowi[j];
+                sr[r][j] -= cr * xr - ci * xi;
+                si[r][j] -= cr * xi + ci * xr;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int j = 0; j < nb; ++j) colsrc[j] = j;
+        for (int k = nb - 1; k >= 0; --k) {
+            const int p = pivs[k];
+            const int t = colsrc[k];
+            colsrc[k] = colsrc[p];
+            colsrc[p] = t;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nb * nb; e += NT) {
+        const int i = e / nb, j = e % nb;
+        const int js = colsrc[j];
+        const T vr = sr[i][js], vi = si[i][js];
+        Are[(int64_t)(p0 + i) * lda + p0 + j] = vr;
+        Aim[(int64_t)(p0 + i) * lda + p0 + j] = vi;
+        Dre[(int64_t)i * ldd + j] = vr;
+        Dim[(int64_t)i * ldd + j] = vi;
+    }
+    if (bad && tid == 0) atomicOr(status, 1);
+}
+
+// Copy the new row panel R (nb x N, rows of Rbuf) into A[p0:p0+nb, :] and the
+// new column panel C (N x nb) into A[:, p0:p0+nb], skipping the diagonal block.
+template <typename T>
+__global__ void gj_writeback_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int p0, int nb, const T* Rre,
+                                    const T* Rim, int64_t ldr, const T* Cre, const T* Cim, int64_t ldc) {
+    const int64_t total = (int64_t)nb * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        // row panel element (i, j)
+        {
+            const int64_t i = e / N, j = e % N;
+            if (j < p0 || j >= p0 + nb) {
+                Are[(p0 + i) * lda + j] = Rre[i * ldr + j];
+                Aim[(p0 + i) * lda + j] = Rim[i * ldr + j];
+            }
+        }
+        // column panel element (i, j)
+        {
+            const int64_t i = e / nb, j = e % nb;
+            if (i < p0 || i >= p0 + nb) {
+                Are[i * lda + p0 + j] = Cre[i * ldc + j];
+                Aim[i * lda + p0 + j] = Cim[i * ldc + j];
+            }
+        }
+    }
+}
+
+template <typename T>
+static constexpr int diag_bmax() {
+    return sizeof(T) == 8 ? 64 : 128;
+}
+
+template <typename T>
+size_t inverse_workspace_elems(int64_t N, int b) {
+    // Rbuf (b x N) + Cbuf (N x b) + Dbuf (b x b), two planes each
+    return (size_t)2 * ((size_t)b * N * 2 + (size_t)b * b);
+}
+
+void KernelTimer::begin(hipStream_t s) {
+    if (used + 2 > ev.size()) {
+        for (int q = 0; q < 64; ++q) {
+            hipEvent_t e;
+            RGF_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+    }
+    RGF_HIP(hipEventRecord(ev[used], s));
+}
+
+void KernelTimer::end(hipStream_t s, double kernel_flops) {
+    RGF_HIP(hipEventRecord(ev[used + 1], s));
+    used += 2;
+    flops += kernel_flops;
+    launches += 1;
+}
+
+double KernelTimer::seconds() {
+    double t = 0;
+    for (size_t q = 0; q + 1 < used; q += 2) {
+        RGF_HIP(hipEventSynchronize(ev[q + 1]));
+        float ms = 0;
+        RGF_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+        t += ms * 1e-3;
+    }
+    return t;
+}
+
+KernelTimer::~KernelTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+template <typename T>
+void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_status, int use3m,
+                    KernelTimer* timer) {
+    if (N <= 0) return;
+    if (b <= 0) b = diag_bmax<T>();
+    if (b > diag_bmax<T>() || (b % 16) != 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "invalid inverse block size");
+    T* Rre = ws;
+    T* Rim = Rre + (size_t)b * N;
+    T* Cre = Rim + (size_t)b * N;
+    T* Cim = Cre + (size_t)N * b;
+    T* Dre = Cim + (size_t)N * b;
+    T* Dim = Dre + (size_t)b * b;
+    CMat<T> R(Rre, Rim, N), C(Cre, Cim, b), D(Dre, Dim, b);
+    for (int64_t p0 = 0; p0 < N; p0 += b) {
+        const int nb = (int)std::min<int64_t>(b, N - p0);
+        constexpr int BMAX = diag_bmax<T>();
+        hipLaunchKernelGGL((gj_diag_kernel<T, BMAX, 512>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb,
+                           D.re, D.im, D.ld, dev_status);
+        if (nb == N) break;  // single block: done
+        // R = D^-1 * A[P, :]
+        cgemm<T>(s, nb, N, nb, D, A.offset(p0, 0), R, cd(1, 0), 0, 0, 0, use3m);
+        // C = -A[:, P] * D^-1 (reads the old column panel)
+        cgemm<T>(s, N, nb, nb, A.offset(0, p0), D, C, cd(-1, 0), 0, 0, 0, use3m);
+        // A[J, J] -= A[J, P] * R[:, J]
+        if (timer) timer->begin(s);
+        cgemm<T>(s, N, N, nb, A.offset(0, p0), R, A, cd(-1, 0), 1, p0, p0 + nb, use3m);
+        if (timer) timer->end(s, (use3m ? 6.0 : 8.0) * (double)(N - nb) * (double)(N - nb) * nb);
+        const int64_t tot = (int64_t)nb * N;
+        const int nblk = (int)std::min<int64_t>((tot + 255) / 256, 4096);
+        hipLaunchKernelGGL((gj_writeback_kernel<T>), dim3(nblk), dim3(256), 0, s, A.re, A.im, A.ld, N, (int)p0, nb,
+                           R.re, R.im, R.ld, C.re, C.im, C.ld);
+    }
+    RGF_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ K1: slice assembly
+template <typename T>
+__global__ void assemble_dense_kernel(int64_t N, T* Mre, T* Mim, int64_t ld, const double* eps_re,
+                                      const double* eps_im, double ecr, double eci, double k2r, double k2i,
+                                      const T* Xre, const T* Xim, int64_t ldx) {
+    const int64_t total = N * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / N, j = e % N;
+        const int64_t o = i * ld + j;
+        double vr = -(double)Mre[o], vi = -(double)Mim[o];
+        if (Xre) {
+            vr -= (double)Xre[i * ldx + j];
+            vi -= (double)Xim[i * ldx + j];
+        }
+        if (i == j) {
+            const double er = eps_re ? eps_re[i] : ecr;
+            const double ei = eps_re ? (eps_im ? eps_im[i] : 0.0) : eci;
+            vr -= k2r * er - k2i * ei;
+            vi -= k2r * ei + k2i * er;
+        }
+        Mre[o] = (T)vr;
+        Mim[o] = (T)vi;
+    }
+}
+
+template <typename T>
+__global__ void csr_add_kernel(int32_t n, const int32_t* rowptr, const int32_t* col, const double* vre,
+                               const double* vim, T* Mre, T* Mim, int64_t ld) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        const int64_t o = (int64_t)i * ld + col[q];
+        Mre[o] = (T)((double)Mre[o] + vre[q]);
+        Mim[o] = (T)((double)Mim[o] + vim[q]);
+    }
+}
+
+static int grid_for(int64_t total, int threads = 256, int cap = 16384) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((total + threads - 1) / threads, cap));
+}
+
+template <typename T>
+void assemble_slice(hipStream_t s, int64_t N, CMat<T> M, const double* eps_re, const double* eps_im,
+                    cd eps_const, cd k2, const CMat<T>* Gextra, const DevCSR& A0) {
+    hipLaunchKernelGGL((assemble_dense_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, M.re, M.im, M.ld,
+                       eps_re, eps_im, eps_const.real(), eps_const.imag(), k2.real(), k2.imag(),
+                       Gextra ? Gextra->re : (const T*)nullptr, Gextra ? Gextra->im : (const T*)nullptr,
+                       Gextra ? Gextra->ld : (int64_t)0);
+    hipLaunchKernelGGL((csr_add_kernel<T>), dim3((A0.n + 255) / 256), dim3(256), 0, s, A0.n, A0.rowptr, A0.col,
+                       A0.vre, A0.vim, M.re, M.im, M.ld);
+    RGF_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ dense helpers
+template <typename T>
+__global__ void fill_kernel(int64_t rows, int64_t cols, T* re, T* im, int64_t ld, int ident) {
+    const int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / cols, j = e % cols;
+        re[i * ld + j] = (ident && i == j) ? T(1) : T(0);
+        im[i * ld + j] = T(0);
+    }
+}
+
+template <typename T>
+void fill_zero(hipStream_t s, int64_t rows, int64_t cols, CMat<T> A) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(rows * cols)), dim3(256), 0, s, rows, cols, A.re, A.im,
+                       A.ld, 0);
+}
+
+template <typename T>
+void set_identity(hipStream_t s, int64_t N, CMat<T> A) {
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, N, A.re, A.im, A.ld, 1);
+}
+
+template <typename Td, typename Ts>
+__global__ void convert_kernel(int64_t rows, int64_t cols, const Ts* sre, const Ts* sim, int64_t lds, Td* dre,
+                               Td* dim, int64_t ldd, double scale) {
+    const int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / cols, j = e % cols;
+        dre[i * ldd + j] = (Td)((double)sre[i * lds + j] * scale);
+        dim[i * ldd + j] = (Td)((double)sim[i * lds + j] * scale);
+    }
+}
+
+template <typename Td, typename Ts>
+void convert(hipStream_t s, int64_t rows, int64_t cols, CMat<Ts> src, CMat<Td> dst, double scale) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL((convert_kernel<Td, Ts>), dim3(grid_for(rows * cols)), dim3(256), 0, s, rows, cols, src.re,
+                       src.im, src.ld, dst.re, dst.im, dst.ld, scale);
+}
+
+template <typename T>
+void copy_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> src, CMat<T> dst) {
+    convert<T, T>(s, rows, cols, src, dst, 1.0);
+}
+
+template <typename T>
+__global__ void accumulate_kernel(int64_t rows, int64_t cols, const T* sre, const T* sim, int64_t lds,
+                                  double* dre, double* dim, int64_t ldd, double scale) {
+    const int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / cols, j = e % cols;
+        dre[i * ldd + j] += (double)sre[i * lds + j] * scale;
+        dim[i * ldd + j] += (double)sim[i * lds + j] * scale;
+    }
+}
+
+template <typename T>
+void accumulate(hipStream_t s, int64_t rows, int64_t cols, CMat<T> src, CMat<double> dst, double scale) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL((accumulate_kernel<T>), dim3(grid_for(rows * cols)), dim3(256), 0, s, rows, cols, src.re,
+                       src.im, src.ld, dst.re, dst.im, dst.ld, scale);
+}
+
+template <typename T>
+__global__ void add_kernel(int64_t rows, int64_t cols, const T* are, const T* aim, int64_t lda, const T* bre,
+                           const T* bim, int64_t ldb, T* dre, T* dim, int64_t ldd) {
+    const int64_t total = rows * cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / cols, j = e % cols;
+        dre[i * ldd + j] = are[i * lda + j] + bre[i * ldb + j];
+        dim[i * ldd + j] = aim[i * lda + j] + bim[i * ldb + j];
+    }
+}
+
+template <typename T>
+void add_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> a, CMat<T> b, CMat<T> dst) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL((add_kernel<T>), dim3(grid_for(rows * cols)), dim3(256), 0, s, rows, cols, a.re, a.im, a.ld,
+                       b.re, b.im, b.ld, dst.re, dst.im, dst.ld);
+}
+
+// ------------------------------------------------------------------ K4: mode matrices
+__global__ void fill_modes_kernel(int64_t n1, int64_t n2, const double* f1, const double* f2, const double* g,
+                                  double* p1r, double* p1i, int64_t ld1, double* p2r, double* p2i, int64_t ld2) {
+    const int64_t N = n1 * n2;
+    const int64_t total = N * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / N, a = e % N;
+        const int64_t i1 = i % n1, i2 = i / n1, a1 = a % n1, a2 = a / n1;
+        const double xr = f1[2 * (i1 * n1 + a1)], xi = f1[2 * (i1 * n1 + a1) + 1];
+        const double yr = f2[2 * (i2 * n2 + a2)], yi = f2[2 * (i2 * n2 + a2) + 1];
+        const double pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
+        double gr = 1.0, gi = 0.0;
+        if (g) {
+            gr = g[2 * a];
+            gi = g[2 * a + 1];
+        }
+        p1r[i * ld1 + a] = pr * gr - pi * gi;
+        p1i[i * ld1 + a] = pr * gi + pi * gr;
+        p2r[a * ld2 + i] = pr;
+        p2i[a * ld2 + i] = -pi;
+    }
+}
+
+void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2, const double* g,
+                CMat<double> P1, CMat<double> P2) {
+    const int64_t N = n1 * n2;
+    hipLaunchKernelGGL(fill_modes_kernel, dim3(grid_for(N * N)), dim3(256), 0, s, n1, n2, f1, f2, g, P1.re, P1.im,
+                       P1.ld, P2.re, P2.im, P2.ld);
+    RGF_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ residual (FP64)
+// psi/r are (L+2) slices of N x m, row-major, slice stride N*m.
+__global__ void residual_kernel(int64_t N, int64_t L, int64_t m, const double* pr, const double* pim, double* rr,
+                                double* ri, const double* er, const double* ei, double eLr, double eLi, double eRr,
+                                double eRi, double k2r, double k2i, const int32_t* rowptr, const int32_t* col,
+                                const double* vre, const double* vim, const double* b0r, const double* b0i,
+                                int64_t mL, const double* b1r, const double* b1i) {
+    const int64_t S = N * m;
+    const int64_t total = (L + 2) * S;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = e / S, rem = e % S, i = rem / m, c = rem % m;
+        // eps of slice n: leads at 0 and L+1, scattering slices 1..L
+        double xr, xi;
+        if (n == 0) {
+            xr = eLr;
+            xi = eLi;
+        } else if (n == L + 1) {
+            xr = eRr;
+            xi = eRi;
+        } else {
+            xr = er[(n - 1) * N + i];
+            xi = ei ? ei[(n - 1) * N + i] : 0.0;
+        }
+        // k2*eps
+        const double ker = k2r * xr - k2i * xi, kei = k2r * xi + k2i * xr;
+        const double psr = pr[e], psi = pim[e];
+        // -(A0 psi)_i + k2 eps psi
+        double accr = ker * psr - kei * psi;
+        double acci = ker * psi + kei * psr;
+        for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+            const int64_t o = n * S + (int64_t)col[q] * m + c;
+            const double ar = vre[q], ai = vim[q];
+            accr -= ar * pr[o] - ai * pim[o];
+            acci -= ar * pim[o] + ai * pr[o];
+        }
+        if (n > 0) {
+            accr += pr[e - S];
+            acci += pim[e - S];
+        }
+        if (n < L + 1) {
+            accr += pr[e + S];
+            acci += pim[e + S];
+        }
+        if (n == 0 && c < mL) {
+            accr += b0r[i * mL + c];
+            acci += b0i[i * mL + c];
+        }
+        if (n == L + 1 && c >= mL) {
+            accr += b1r[i * (m - mL) + (c - mL)];
+            acci += b1i[i * (m - mL) + (c - mL)];
+        }
+        rr[e] = accr;
+        ri[e] = acci;
+    }
+}
+
+void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const double* psi_re, const double* psi_im,
+                      double* r_re, double* r_im, const double* eps_re, const double* eps_im, cd epsL, cd epsR,
+                      cd k2, const DevCSR& A0, const double* b0_re, const double* b0_im, int64_t mL,
+                      const double* b1_re, const double* b1_im, int64_t, int) {
+    const int64_t total = (L + 2) * N * m;
+    hipLaunchKernelGGL(residual_kernel, dim3(grid_for(total)), dim3(256), 0, s, N, L, m, psi_re, psi_im, r_re, r_im,
+                       eps_re, eps_im, epsL.real(), epsL.imag(), epsR.real(), epsR.imag(), k2.real(), k2.imag(),
+                       A0.rowptr, A0.col, A0.vre, A0.vim, b0_re, b0_im, mL, b1_re, b1_im);
+    RGF_HIP(hipGetLastError());
+}
+
+__global__ void max_abs_kernel(int64_t count, const double* re, const double* im, unsigned long long* out) {
+    double m = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const double a = re[e], b = im ? im[e] : 0.0;
+        const double v = sqrt(a * a + b * b);
+        if (!(v <= m)) m = v;  // NaN propagates
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(m, off);
+        if (!(o <= m)) m = o;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long bits = isnan(m) ? 0x7ff8000000000000ULL : (unsigned long long)__double_as_longlong(m);
+        atomicMax(out, bits);
+    }
+}
+
+double max_abs(hipStream_t s, int64_t count, const double* re, const double* im, double* dev_scratch) {
+    if (count <= 0) return 0.0;
+    RGF_HIP(hipMemsetAsync(dev_scratch, 0, sizeof(double), s));
+    hipLaunchKernelGGL(max_abs_kernel, dim3(grid_for(count, 256, 2048)), dim3(256), 0, s, count, re, im,
+                       (unsigned long long*)dev_scratch);
+    double h = 0;
+    RGF_HIP(hipMemcpyAsync(&h, dev_scratch, sizeof(double), hipMemcpyDeviceToHost, s));
+    RGF_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// ------------------------------------------------------------------ instantiations
+#define RGF_INST(T)                                                                                          \
+    template void cgemm<T>(hipStream_t, int64_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>, cd, int, int64_t, \
+                           int64_t, int);                                                                     \
+    template size_t inverse_workspace_elems<T>(int64_t, int);                                                 \
+    template void invert_inplace<T>(hipStream_t, int64_t, CMat<T>, int, T*, int*, int, KernelTimer*);         \
+    template void assemble_slice<T>(hipStream_t, int64_t, CMat<T>, const double*, const double*, cd, cd,      \
+                                    const CMat<T>*, const DevCSR&);                                           \
+    template void fill_zero<T>(hipStream_t, int64_t, int64_t, CMat<T>);                                       \
+    template void set_identity<T>(hipStream_t, int64_t, CMat<T>);                                             \
+    template void copy_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>);                               \
+    template void accumulate<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<double>, double);                \
+    template void add_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>);
+
+RGF_INST(double)
+RGF_INST(float)
+template void convert<float, double>(hipStream_t, int64_t, int64_t, CMat<double>, CMat<float>, double);
+template void convert<double, float>(hipStream_t, int64_t, int64_t, CMat<float>, CMat<double>, double);
+template void convert<double, double>(hipStream_t, int64_t, int64_t, CMat<double>, CMat<double>, double);
+template void convert<float, float>(hipStream_t, int64_t, int64_t, CMat<float>, CMat<float>, double);
+
+}  // namespace rgf

@@ -1,0 +1,187 @@
+"""GPU parity of librgf against the FP64 oracle and the golden fixtures.
+
+Tolerances: complex FP64 path <= 1e-10 max-entry error against the oracle's
+S (entries are O(1)); FP32+refinement path <= 1e-9.  Kernel-level checks
+compare the MFMA GEMM and the slice-inverse kernel with numpy.
+"""
+import ast
+import os
+
+import numpy as np
+import pytest
+
+import oracle.rgf_oracle as o
+import rgf_amd
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rgf_golden.npz")
+TOL64 = 1e-10
+TOL_MIXED = 1e-9
+
+
+def _crand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32_refined", 2e-6)])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200), (300, 257, 129),
+                                 (513, 1100, 96)])
+def test_mfma_complex_gemm(precision, tol, mnk):
+    m, n, k = mnk
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    A, B = _crand(rng, m, k), _crand(rng, k, n)
+    Cm = rgf_amd.dense_gemm(A, B, precision=precision)
+    ref = A @ B
+    scale = np.abs(A).max() * np.abs(B).max() * k
+    assert np.abs(Cm - ref).max() / scale < tol
+
+
+def _well_posed(rng, n):
+    return _crand(rng, n, n) / np.sqrt(n) + 4 * np.eye(n)
+
+
+@pytest.mark.parametrize("n,b", [(1, 0), (5, 0), (64, 0), (100, 16), (200, 32), (333, 64), (700, 0)])
+def test_slice_inverse_fp64(n, b):
+    rng = np.random.default_rng(n + b)
+    A = _well_posed(rng, n)
+    X = rgf_amd.dense_inverse(A, "fp64", b)
+    assert np.abs(X @ A - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,b", [(7, 0), (128, 0), (300, 64), (1000, 128)])
+def test_slice_inverse_fp32(n, b):
+    rng = np.random.default_rng(n + b + 1)
+    A = _well_posed(rng, n)
+    X = rgf_amd.dense_inverse(A, "fp32_refined", b)
+    assert np.abs(X @ A - np.eye(n)).max() < 5e-5
+
+
+def test_slice_inverse_pivots_inside_block():
+    # zero diagonal: needs row interchanges within the diagonal block
+    n = 48
+    rng = np.random.default_rng(3)
+    P = np.eye(n)[rng.permutation(n)]
+    A = P @ (np.eye(n) * 3 + 0.1 * _crand(rng, n, n))
+    A[np.arange(n), np.arange(n)] = 0.0
+    X = rgf_amd.dense_inverse(A, "fp64", 64)
+    assert np.abs(X - np.linalg.inv(A)).max() < 1e-12
+
+
+def _cells(s):
+    return [c for c in s]
+
+
+def _load_golden():
+    z = np.load(GOLDEN, allow_pickle=False)
+    return z, [str(x) for x in z["names"]]
+
+
+def _golden_case(z, name):
+    eps = z[f"{name}/eps"]
+    k0dx, eL, eR = z[f"{name}/params"]
+    bc = str(z[f"{name}/bc"])
+    out, inn = str(z[f"{name}/sel"]).split("|")
+    S = z[f"{name}/S"]
+    k0dx = k0dx.real if k0dx.imag == 0 else k0dx
+    eL = eL.real if eL.imag == 0 else eL
+    eR = eR.real if eR.imag == 0 else eR
+    return eps, k0dx, eL, eR, bc, out, inn, S
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
+def test_golden_fixtures(precision, tol):
+    z, names = _load_golden()
+    for name in names:
+        eps, k0dx, eL, eR, bc, out, inn, Sref = _golden_case(z, name)
+        if eps.ndim == 3:
+            xy = bc if bc in ("periodic", "Dirichlet") else ast.literal_eval(bc)
+            syst = dict(epsilon=eps, epsilon_L=eL, epsilon_R=eR, k0dx=k0dx, xyBC=xy)
+            S, _ = rgf_amd.cal_smatrix_RGF_3D(syst, _cells(out), _cells(inn), precision=precision)
+        else:
+            try:
+                yb = float(bc)
+            except ValueError:
+                yb = bc
+            syst = dict(epsilon=eps, epsilon_L=eL, epsilon_R=eR, k0dx=k0dx, yBC=yb)
+            S, _ = rgf_amd.cal_smatrix_RGF(syst, _cells(out), _cells(inn), precision=precision)
+        assert S.shape == Sref.shape, name
+        err = np.abs(S - Sref).max()
+        print(f"{name} {precision} err={err:.2e}", flush=True)
+        assert err < tol, (name, err)
+
+
+SELECTIONS = [
+    (["L", "R"], ["L", "R"]), (["L"], ["L"]), (["R"], ["R"]), (["R"], ["L"]), (["L"], ["R"]),
+    (["L", "R"], ["L"]), (["L", "R"], ["R"]), (["R", "L"], ["L"]),
+]
+
+
+@pytest.mark.parametrize("bc", ["periodic", "Dirichlet", "Neumann", "DirichletNeumann", "NeumannDirichlet", 1.1])
+@pytest.mark.parametrize("sel", range(len(SELECTIONS)))
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
+def test_smatrix_2d_schemes(bc, sel, precision, tol):
+    out, inn = SELECTIONS[sel]
+    rng = np.random.default_rng(sel * 31 + len(str(bc)))
+    ny, nx = 37, 23
+    eps = 1 + 2 * rng.random((ny, nx)) + 0.02j * rng.random((ny, nx))
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.69, k0dx=2 * np.pi / 6.5, yBC=bc)
+    S, _ = rgf_amd.cal_smatrix_RGF(syst, out, inn, precision=precision)
+    Sref, _ = o.cal_smatrix_RGF(syst, out, inn)
+    assert np.abs(S - Sref).max() < tol
+
+
+def test_struct_selection_and_delta():
+    rng = np.random.default_rng(9)
+    syst = dict(epsilon=1 + rng.random((30, 10)), epsilon_L=1.0, epsilon_R=1.0, k0dx=1.0, yBC="periodic")
+    out = {"ind_out_L": [3, 1, 3], "ind_out_R": [2, 5]}
+    inn = {"ind_in_L": [1, 3], "ind_in_R": [5, 2, 1]}
+    S, _ = rgf_amd.cal_smatrix_RGF(syst, out, inn)
+    Sref, _ = o.cal_smatrix_RGF(syst, out, inn)
+    assert np.abs(S - Sref).max() < TOL64
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
+def test_smatrix_3d(precision, tol):
+    rng = np.random.default_rng(21)
+    for bc in ["periodic", "Dirichlet", [0.7, 2.9]]:
+        nx, ny, nz = 6, 5, 7
+        eps = 1 + 1.1 * rng.random((nx, ny, nz))
+        syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.3, k0dx=1.2, xyBC=bc)
+        for out, inn in (["L", "R"], ["L", "R"]), (["R"], ["L"]):
+            S, _ = rgf_amd.cal_smatrix_RGF_3D(syst, out, inn, precision=precision)
+            Sref, _ = o.cal_smatrix_RGF_3D(syst, out, inn)
+            assert np.abs(S - Sref).max() < tol
+
+
+def _cfg1(seed=1):
+    rng = np.random.default_rng(seed)
+    eps = 1 + 3 * rng.random((64, 200))
+    return dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.0, k0dx=2 * np.pi / 15, yBC="periodic")
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
+def test_cfg1_full_size(precision, tol):
+    """2D N=64, 200 slices, periodic, eps~U[1,4], lambda=15 px, full S (18 x 18)."""
+    syst = _cfg1()
+    S, ch, info = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"], precision=precision, return_info=True)
+    Sref, _ = o.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"])
+    assert S.shape == (18, 18)
+    assert np.abs(S - Sref).max() < tol
+    assert np.abs(S.conj().T @ S - np.eye(18)).max() < 1e-9
+    perm = np.concatenate([ch["L"]["ind_prop_conj"], ch["L"]["N_prop"] + ch["R"]["ind_prop_conj"]]) - 1
+    St = S[perm]
+    assert np.abs(St.T - St).max() < 1e-9
+    print(info, flush=True)
+
+
+def test_wide_slab_mixed_vs_fp64():
+    """2D N=1024 Dirichlet slab with absorption and eps_R != eps_L (cfg-2 shape, 40 slices)."""
+    rng = np.random.default_rng(2)
+    eps = 1 + 1.25 * (rng.random((1024, 40)) < 0.3) + 0.01j
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=2.25, k0dx=2 * np.pi / 10, yBC="Dirichlet")
+    S64, ch = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"])
+    S32, _ = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"], precision="fp32_refined")
+    assert S64.shape == (208 + 320, 208 + 320)
+    assert np.abs(S32 - S64).max() < TOL_MIXED
+    assert np.linalg.eigvalsh(S64.conj().T @ S64).max() <= 1 + 1e-9
