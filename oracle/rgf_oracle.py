@@ -562,3 +562,69 @@ def transmission_gradient_fd(syst: dict, pixels, h: float = 1e-6, three_d: bool 
             vals.append(float(np.sum(np.abs(S) ** 2)))
         out.append((vals[0] - vals[1]) / (2 * h))
     return np.array(out)
+
+
+def transmission_gradient(syst: dict, three_d: bool = False):
+    """T = sum |t_ij|^2 (inputs L, outputs R) and dT/dRe(eps), dT/dIm(eps), FP64.
+
+    Independent of the slice recursion: the forward fields Psi = H^-1 [B_L at
+    slice 0] and the adjoint rows Phi = C_R H^-1 (row of slice L+1) come from
+    one sparse LU of the whole block-tridiagonal system H (slices 0..L+1, leads
+    folded into the end blocks).  With t = -2i C_R G_{L+1,0} B_L and
+    dH_nn = -k0dx^2 d(eps) on the diagonal,
+        dt_ij / d eps_{m,n} = -2i k0dx^2 Phi_n[i,m] Psi_n[m,j]
+    and for g = sum_ij conj(t_ij) dt_ij/d eps:  dT/dRe eps = 2 Re g,
+    dT/dIm eps = -2 Im g.
+    """
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    eps = np.asarray(syst["epsilon"])
+    if three_d:
+        nx, ny, L = eps.shape
+        ch = setup_channels_3D(nx, ny, syst["xyBC"], syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"])
+        A0 = slice_operator_3d(nx, ny, syst["xyBC"])
+        slices = [eps[:, :, n].ravel(order="F") for n in range(L)]
+    else:
+        ny, L = eps.shape
+        ch = setup_channels(ny, syst["yBC"], syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"])
+        A0 = slice_operator_2d(ny, syst["yBC"])
+        slices = [eps[:, n] for n in range(L)]
+    N = A0.shape[0]
+    k2 = complex(syst["k0dx"]) ** 2
+    phi = ch["phi"]
+    G0L = lead_greens_function(phi, ch["L"]["kxdx"])
+    G0R = lead_greens_function(phi, ch["R"]["kxdx"])
+    blocks = [[None] * (L + 2) for _ in range(L + 2)]
+    I = sp.identity(N, dtype=complex, format="csr")
+    blocks[0][0] = sp.csr_matrix(A0 - k2 * syst["epsilon_L"] * np.eye(N) - G0L)
+    for n in range(L):
+        blocks[n + 1][n + 1] = sp.csr_matrix(A0 - np.diag(k2 * slices[n]))
+    blocks[L + 1][L + 1] = sp.csr_matrix(A0 - k2 * syst["epsilon_R"] * np.eye(N) - G0R)
+    for n in range(L + 1):
+        blocks[n][n + 1] = -I
+        blocks[n + 1][n] = -I
+    H = sp.bmat(blocks, format="csc")
+    L_, R_ = ch["L"], ch["R"]
+    B = phi[:, L_["ind_prop"] - 1] * L_["sqrt_mu"][None, :]
+    C = R_["sqrt_mu"][:, None] * phi[:, R_["ind_prop"] - 1].conj().T
+    lu = spla.splu(H)
+    rhs = np.zeros((N * (L + 2), B.shape[1]), complex)
+    rhs[:N] = B
+    Psi = lu.solve(rhs)
+    rhs2 = np.zeros((N * (L + 2), C.shape[0]), complex)
+    rhs2[N * (L + 1):] = C.T
+    PhiT = lu.solve(rhs2, trans="T")          # (C H^-1)^T = H^-T C^T
+    t = -2j * (C @ Psi[N * (L + 1):])
+    T = float(np.sum(np.abs(t) ** 2))
+    gre = np.zeros((N, L))
+    gim = np.zeros((N, L))
+    for n in range(L):
+        Ps = Psi[N * (n + 1):N * (n + 2)]          # N x nin
+        Ph = PhiT[N * (n + 1):N * (n + 2)].T       # nout x N
+        g = -2j * k2 * np.einsum("mj,ij,im->m", Ps, t.conj(), Ph)
+        gre[:, n] = 2 * g.real
+        gim[:, n] = -2 * g.imag
+    if three_d:
+        gre = gre.reshape((nx, ny, L), order="F")
+        gim = gim.reshape((nx, ny, L), order="F")
+    return T, gre, gim

@@ -13,9 +13,11 @@ from .api import (  # noqa: F401
     dense_inverse,
     setup_channels,
     setup_channels_3D,
+    transmission_gradient,
 )
 
 __all__ = [
     "cal_smatrix_RGF", "cal_smatrix_RGF_3D", "setup_channels", "setup_channels_3D",
     "build_laplacian_1d", "load_library", "RGFError", "dense_inverse", "dense_gemm",
+    "transmission_gradient",
 ]

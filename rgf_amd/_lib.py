@@ -117,6 +117,15 @@ def load_library():
     global _lib
     with _lock:
         if _lib is None:
+            # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 with
+            # the same SONAMEs as the system ROCm; whichever is loaded first
+            # serves the whole process, and torch cannot start on a runtime
+            # another library initialised first.  Let torch initialise first.
+            try:
+                import torch
+                torch.cuda.is_available()
+            except ImportError:
+                pass
             if not os.path.exists(LIB_PATH):
                 raise ImportError(f"{LIB_PATH} is not built; run `python -m rgf_amd.build`")
             lib = C.CDLL(LIB_PATH)

@@ -1,0 +1,88 @@
+"""torch.autograd.Function for the transmission T = sum |t_ij|^2 and its adjoint.
+
+The forward pass and the gradient come from one call of
+``rgf_transmission_gradient`` in librgf (the hand-written adjoint of the slice
+recursion); CUDA tensors are handed to the library as device pointers and
+the library runs on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .api import _bc_1d, _bc_xy, _options, _parse_selection, _Prob, _query, _scalar, _Sel
+
+
+def _problem_from_tensor(eps: torch.Tensor, epsilon_L, epsilon_R, k0dx, bc, three_d: bool):
+    """rgf_problem over a tensor in MATLAB index order (ny x nx, or nx x ny x nz)."""
+    if three_d:
+        nx, ny, nz = eps.shape
+        dims = (nx, ny, nz)
+        bcs, kas = _bc_xy(bc)
+        cm = eps.detach().permute(2, 1, 0)            # column-major view: slices contiguous
+    else:
+        ny, nx = eps.shape
+        dims = (ny, nx, 1)
+        b, ka = _bc_1d(bc)
+        bcs, kas = [b], [ka]
+        cm = eps.detach().t()
+    prob = _Prob(3 if three_d else 2, dims, None, _scalar(epsilon_L, "epsilon_L"), _scalar(epsilon_R, "epsilon_R"),
+                 _scalar(k0dx, "k0dx"), bcs, kas)
+    re = cm.real.to(torch.float64).contiguous() if cm.is_complex() else cm.to(torch.float64).contiguous()
+    prob.keep.append(re)
+    prob.p.eps_re = re.data_ptr()
+    if cm.is_complex():
+        im = cm.imag.to(torch.float64).contiguous()
+        prob.keep.append(im)
+        prob.p.eps_im = im.data_ptr()
+    prob.p.eps_on_device = 1 if eps.is_cuda else 0
+    return prob
+
+
+class TransmissionFunction(torch.autograd.Function):
+    """T(eps) = sum_ij |t_ij|^2; backward returns dT/d eps (real eps) or
+    dT/dRe + i dT/dIm (complex eps, PyTorch's convention for real losses)."""
+
+    @staticmethod
+    def forward(ctx, eps, epsilon_L, epsilon_R, k0dx, bc, out, in_, three_d, checkpoint_every):
+        lib = L.load_library()
+        prob = _problem_from_tensor(eps, epsilon_L, epsilon_R, k0dx, bc, three_d)
+        nprop = _query(prob)
+        sel = _Sel(_parse_selection(list(out), "out", nprop), _parse_selection(list(in_), "in", nprop))
+        shape = tuple(eps.shape)
+        n = int(np.prod(shape))
+        gre = np.zeros(n)
+        gim = np.zeros(n)
+        T = C.c_double()
+        dev = eps.device.index if eps.is_cuda else -1
+        h = L.handle(dev if dev is not None else -1)
+        if eps.is_cuda:
+            L.check(lib.rgf_set_stream(h, C.c_void_p(torch.cuda.current_stream(eps.device).cuda_stream)), h)
+        try:
+            st = lib.rgf_transmission_gradient(h, C.byref(prob.p), C.byref(sel.s), C.byref(_options("fp64")),
+                                               int(checkpoint_every), C.byref(T), L.dptr(gre), L.dptr(gim), None)
+        finally:
+            if eps.is_cuda:
+                lib.rgf_set_stream(h, None)
+        L.check(st, h)
+        g = gre.reshape(shape, order="F")
+        if eps.is_complex():
+            g = g + 1j * gim.reshape(shape, order="F")
+        ctx.save_for_backward(torch.as_tensor(g, dtype=eps.dtype if eps.is_complex() else torch.float64,
+                                              device=eps.device))
+        return torch.tensor(T.value, dtype=torch.float64, device=eps.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return (grad_out * g).to(g.dtype), None, None, None, None, None, None, None, None
+
+
+def transmission(eps: torch.Tensor, epsilon_L, epsilon_R, k0dx, bc, out=("R",), in_=("L",), three_d: bool = False,
+                 checkpoint_every: int = 32) -> torch.Tensor:
+    """Differentiable T = sum |t_ij|^2 of the slab eps (MATLAB index order)."""
+    return TransmissionFunction.apply(eps, epsilon_L, epsilon_R, k0dx, bc, tuple(out), tuple(in_), three_d,
+                                      checkpoint_every)

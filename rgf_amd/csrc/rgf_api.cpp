@@ -246,12 +246,14 @@ std::vector<cd> make_C(const Channels& ch, const Side& s, const std::vector<int3
     return C;
 }
 
-void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Selection& sel, Work& w) {
+void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Selection& sel, Work& w,
+             int force_mirror = -1) {
     w.ch = &ch;
     w.N = ch.N;
     w.L = ch.L_slices;
     w.k2 = ch.k2;
-    w.mirrored = (sel.inR.size() + sel.outR.size()) < (sel.inL.size() + sel.outL.size());
+    w.mirrored = force_mirror >= 0 ? force_mirror == 1
+                                   : (sel.inR.size() + sel.outR.size()) < (sel.inL.size() + sel.outL.size());
     if (!w.mirrored) {
         w.sL = &ch.L;
         w.sR = &ch.R;
@@ -294,6 +296,13 @@ void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Sel
     if (!w.outR.empty()) upload(c, make_C(ch, *w.sR, w.outR), w.outR.size(), N, w.CR);
 }
 
+// split-K workspace for the thin GEMMs (elements of T)
+template <typename T>
+std::pair<T*, size_t> splitk_ws(rgf_context* c) {
+    const size_t elems = (size_t)(64u << 20) / sizeof(T);  // 64 MiB
+    return {dbuf<T>(c, sizeof(T) == 8 ? "splitk64" : "splitk32", elems), elems};
+}
+
 int default_block(int precision) { return precision == RGF_PRECISION_FP64 ? 64 : 128; }
 
 // ---------------------------------------------------------------- FP64 carry driver
@@ -307,6 +316,7 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     CMat<double> X = dmat<double>(c, "X", N, pi), X2 = dmat<double>(c, "X2", N, pi);
     CMat<double> Y = dmat<double>(c, "Y", po, N), Y2 = dmat<double>(c, "Y2", po, N);
     CMat<double> Z = dmat<double>(c, "Z", po, pi);
+    auto wk = splitk_ws<double>(c);
     copy_mat<double>(s, N, N, w.G0L, G);
     if (pi) cgemm<double>(s, N, pi, N, w.G0L, w.BL, X, cd(1, 0), 0);
     if (po) cgemm<double>(s, po, N, N, w.CL, w.G0L, Y, cd(1, 0), 0);
@@ -322,12 +332,12 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
             assemble_slice<double>(s, N, G, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
         invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer);
         if (pi) {
-            cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0);
+            cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             std::swap(X, X2);
         }
-        if (po && pi) cgemm<double>(s, po, pi, N, Y, X, Z, cd(1, 0), 1);
+        if (po && pi) cgemm<double>(s, po, pi, N, Y, X, Z, cd(1, 0), 1, 0, 0, 0, wk.first, wk.second);
         if (po) {
-            cgemm<double>(s, po, N, N, Y, G, Y2, cd(1, 0), 0);
+            cgemm<double>(s, po, N, N, Y, G, Y2, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             std::swap(Y, Y2);
         }
     }
@@ -407,6 +417,8 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     double bmax = std::max(max_abs(s, N * mL, w.BL.re, w.BL.im, c->d_scratch),
                            max_abs(s, N * mR, w.BR.re, w.BR.im, c->d_scratch));
     if (bmax == 0) bmax = 1;
+    auto wk = splitk_ws<float>(c);
+    auto wk64 = splitk_ws<double>(c);
     const int maxit = o.max_refine_iter > 0 ? o.max_refine_iter : 6;
     const double tol = o.refine_tol > 0 ? o.refine_tol : 1e-13;
     double rel = 1.0;
@@ -415,8 +427,8 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         // r = b - H psi
         residual_stencil(s, N, L, m, psi, psi + S * SE, r, r + S * SE, w.eps_re, w.eps_im, w.epsL, w.epsR, w.k2, w.A0,
                          w.BL.re, w.BL.im, mL, w.BR.re, w.BR.im, 0, 0);
-        cgemm<double>(s, N, m, N, w.G0L, P(psi, 0), P(r, 0), cd(1, 0), 1);
-        cgemm<double>(s, N, m, N, w.G0R, P(psi, L + 1), P(r, L + 1), cd(1, 0), 1);
+        cgemm<double>(s, N, m, N, w.G0L, P(psi, 0), P(r, 0), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
+        cgemm<double>(s, N, m, N, w.G0R, P(psi, L + 1), P(r, L + 1), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
         const double rmax = max_abs(s, S * SE, r, r + S * SE, c->d_scratch);
         rel = rmax / bmax;
         if (!std::isfinite(rel)) throw Error(RGF_ERR_NUMERICAL, "non-finite residual in refinement");
@@ -426,10 +438,11 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         for (int64_t n = 0; n < S; ++n) {
             convert<float, double>(s, N, m, P(r, n), t, sc);
             if (n > 0) add_mat<float>(s, N, m, t, D(n - 1), t);
-            cgemm<float>(s, N, m, N, Fn(n), t, D(n), cd(1, 0), 0, 0, 0, use3m);
+            cgemm<float>(s, N, m, N, Fn(n), t, D(n), cd(1, 0), 0, 0, 0, use3m, wk.first, wk.second);
         }
         // backward sweep: d_n += g_n d_{n+1}
-        for (int64_t n = S - 2; n >= 0; --n) cgemm<float>(s, N, m, N, Fn(n), D(n + 1), D(n), cd(1, 0), 1, 0, 0, use3m);
+        for (int64_t n = S - 2; n >= 0; --n)
+            cgemm<float>(s, N, m, N, Fn(n), D(n + 1), D(n), cd(1, 0), 1, 0, 0, use3m, wk.first, wk.second);
         accumulate<float>(s, S * N, m, CMat<float>(d, d + S * SE, m), CMat<double>(psi, psi + S * SE, m), rmax);
     }
     w.refine_iters = it;
@@ -465,6 +478,174 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         snprintf(buf, sizeof buf, "mixed-precision refinement stopped at relative residual %.3e (tolerance %.1e)", rel,
                  tol);
         throw Error(RGF_ERR_NOT_CONVERGED, buf);
+    }
+}
+
+void fill_info(rgf_info* info, const Channels& ch, const Selection& sel) {
+    if (!info) return;
+    memset(info, 0, sizeof *info);
+    info->n_transverse = ch.N;
+    info->n_slices = ch.L_slices;
+    info->n_prop_L = ch.L.n_prop;
+    info->n_prop_R = ch.R.n_prop;
+    info->n_out = (int32_t)(sel.outL.size() + sel.outR.size());
+    info->n_in = (int32_t)(sel.inL.size() + sel.inR.size());
+}
+
+// ---------------------------------------------------------------- adjoint gradient (FP64)
+// T = sum_ij |t_ij|^2 with t = -2i C_R G_{L+1,0} B_L (sweep-side names).
+// Forward: g_n = (D_n - g_{n-1})^-1 and z_n = g_n z_{n-1} (z_0 = G0_L B_L);
+// the pairs (g_n, z_n) are kept at every `every`-th slice.  Backward, with
+// each segment recomputed from its checkpoint:
+//   psi_n = z_n + g_n psi_{n+1}    (psi_{L+1} = z_{L+1}: forward fields)
+//   phi_n = phi_{n+1} g_n           (phi_{L+1} = t^H C_R g_{L+1}: adjoint rows)
+// and dT/d eps_{m,n} from g = -2i k^2 sum_j psi_n[m,j] phi_n[j,m].
+void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<double>& gre, std::vector<double>& gim) {
+    const int64_t N = w.N, L = w.L, m = w.inL.size(), no = w.outR.size();
+    const int b = default_block(RGF_PRECISION_FP64);
+    hipStream_t s = c->stream;
+    const int64_t C = (every <= 0 || every > L + 1) ? L + 2 : every;
+    const int64_t nck = L / C + 1;  // checkpoints at n = 0, C, 2C, ... <= L
+    const size_t NN = (size_t)N * N, NM = (size_t)N * m;
+    double* ckg = dbuf<double>(c, "ck_g", 2 * NN * nck);
+    double* ckz = dbuf<double>(c, "ck_z", 2 * NM * nck);
+    double* sg = dbuf<double>(c, "seg_g", 2 * NN * std::max<int64_t>(C, 1));
+    double* sz = dbuf<double>(c, "seg_z", 2 * NM * std::max<int64_t>(C, 1));
+    auto mat = [&](double* base, size_t per, int64_t k, int64_t cols) {
+        return CMat<double>(base + 2 * per * k, base + 2 * per * k + per, cols);
+    };
+    auto slot_g = [&](int64_t n) { return (n % C == 0) ? mat(ckg, NN, n / C, N) : mat(sg, NN, n % C, N); };
+    auto slot_z = [&](int64_t n) { return (n % C == 0) ? mat(ckz, NM, n / C, m) : mat(sz, NM, n % C, m); };
+    CMat<double> G = dmat<double>(c, "G", N, N);
+    CMat<double> za = dmat<double>(c, "za", N, m), zb = dmat<double>(c, "zb", N, m);
+    double* ws = dbuf<double>(c, "inv_ws", inverse_workspace_elems<double>(N, b));
+    auto wk = splitk_ws<double>(c);
+    auto dyson = [&](int64_t n, CMat<double> prev, CMat<double> out) {
+        copy_mat<double>(s, N, N, prev, out);
+        if (n <= L)
+            assemble_slice<double>(s, N, out, w.eps_re + (n - 1) * N, w.eps_im ? w.eps_im + (n - 1) * N : nullptr,
+                                   cd(0, 0), w.k2, nullptr, w.A0);
+        else
+            assemble_slice<double>(s, N, out, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
+        invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer);
+    };
+    reset_status(c);
+    RGF_HIP(hipStreamSynchronize(s));
+    const double t0 = now();
+    // forward sweep
+    copy_mat<double>(s, N, N, w.G0L, slot_g(0));
+    cgemm<double>(s, N, m, N, w.G0L, w.BL, slot_z(0), cd(1, 0), 0);
+    copy_mat<double>(s, N, N, w.G0L, G);
+    copy_mat<double>(s, N, m, slot_z(0), za);
+    for (int64_t n = 1; n <= L + 1; ++n) {
+        dyson(n, G, G);
+        cgemm<double>(s, N, m, N, G, za, zb, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
+        std::swap(za, zb);
+        if (n <= L && n % C == 0) {
+            copy_mat<double>(s, N, N, G, slot_g(n));
+            copy_mat<double>(s, N, m, za, slot_z(n));
+        }
+    }
+    // t = -2i C_R z_{L+1}
+    CMat<double> Tm = dmat<double>(c, "T", no, m);
+    cgemm<double>(s, no, m, N, w.CR, za, Tm, cd(0, -2), 0);
+    std::vector<cd> t = download(c, Tm, no, m);
+    T = 0;
+    for (auto& v : t) T += std::norm(v);
+    // phi_{L+1} = t^H C_R g_{L+1} (m x N)
+    std::vector<cd> tH((size_t)m * no);
+    for (int64_t i = 0; i < no; ++i)
+        for (int64_t j = 0; j < m; ++j) tH[j * no + i] = std::conj(t[i * m + j]);
+    CMat<double> TH = dmat<double>(c, "TH", m, no), W = dmat<double>(c, "W", m, N);
+    upload(c, tH, m, no, TH);
+    CMat<double> pha = dmat<double>(c, "pha", m, N), phb = dmat<double>(c, "phb", m, N);
+    cgemm<double>(s, m, N, no, TH, w.CR, W, cd(1, 0), 0);
+    cgemm<double>(s, m, N, N, W, G, pha, cd(1, 0), 0);
+    CMat<double> psa = dmat<double>(c, "psa", N, m), psb = dmat<double>(c, "psb", N, m);
+    copy_mat<double>(s, N, m, za, psa);
+    double* dg = dbuf<double>(c, "grad", 2 * (size_t)N * std::max<int64_t>(L, 1));
+    // backward sweep
+    int64_t cur = -1;
+    for (int64_t n = L; n >= 1; --n) {
+        const int64_t k = n / C;
+        if (k != cur) {
+            for (int64_t q = k * C + 1; q <= std::min(k * C + C - 1, L); ++q) {
+                dyson(q, slot_g(q - 1), slot_g(q));
+                cgemm<double>(s, N, m, N, slot_g(q), slot_z(q - 1), slot_z(q), cd(1, 0), 0, 0, 0, 0, wk.first,
+                              wk.second);
+            }
+            cur = k;
+        }
+        copy_mat<double>(s, N, m, slot_z(n), psb);
+        cgemm<double>(s, N, m, N, slot_g(n), psa, psb, cd(1, 0), 1, 0, 0, 0, wk.first, wk.second);
+        std::swap(psa, psb);
+        cgemm<double>(s, m, N, N, pha, slot_g(n), phb, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
+        std::swap(pha, phb);
+        grad_contract(s, N, m, psa, pha, w.k2, dg + (n - 1) * N, dg + (size_t)N * L + (n - 1) * N);
+    }
+    RGF_HIP(hipStreamSynchronize(s));
+    w.t_slices = now() - t0;
+    check_status(c);
+    const double Nd = (double)N;
+    const int64_t recomputed = L - (L / C);
+    w.flops = (double)(L + 1 + recomputed) * (8 * Nd * Nd * Nd + 8 * Nd * Nd * m) + (double)L * 16 * Nd * Nd * m;
+    gre.assign((size_t)N * L, 0);
+    gim.assign((size_t)N * L, 0);
+    if (L > 0) {
+        RGF_HIP(hipMemcpyAsync(gre.data(), dg, sizeof(double) * N * L, hipMemcpyDeviceToHost, s));
+        RGF_HIP(hipMemcpyAsync(gim.data(), dg + (size_t)N * L, sizeof(double) * N * L, hipMemcpyDeviceToHost, s));
+        RGF_HIP(hipStreamSynchronize(s));
+    }
+}
+
+void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt, int every,
+                  double* T_out, double* gre_out, double* gim_out, rgf_info* info) {
+    const double t0 = now();
+    rgf_options o;
+    rgf_default_options(&o);
+    if (opt) o = *opt;
+    if (o.precision != RGF_PRECISION_FP64)
+        throw Error(RGF_ERR_INVALID_ARGUMENT, "the transmission gradient is computed in FP64");
+    Channels ch = setup_channels(p);
+    Selection sel = make_selection(s, ch);
+    fill_info(info, ch, sel);
+    const bool l2r = !sel.inL.empty() && sel.inR.empty() && !sel.outR.empty() && sel.outL.empty();
+    const bool r2l = sel.inL.empty() && !sel.inR.empty() && sel.outR.empty() && !sel.outL.empty();
+    if (!l2r && !r2l)
+        throw Error(RGF_ERR_INVALID_ARGUMENT,
+                    "the transmission gradient needs inputs on one side and outputs on the other side");
+    RGF_HIP(hipSetDevice(c->device));
+    Work w;
+    std::unique_ptr<KernelTimer> timer;
+    if (o.profile) {
+        timer.reset(new KernelTimer);
+        w.timer = timer.get();
+    }
+    prepare(c, p, ch, sel, w, r2l ? 1 : 0);
+    w.t_setup = now() - t0;
+    double T = 0;
+    std::vector<double> gre, gim;
+    run_gradient(c, w, every, T, gre, gim);
+    if (T_out) *T_out = T;
+    const int64_t N = w.N, L = w.L;
+    for (int64_t n = 0; n < L; ++n) {
+        const int64_t un = w.mirrored ? L - 1 - n : n;  // user slice index
+        for (int64_t i = 0; i < N; ++i) {
+            if (gre_out) gre_out[un * N + i] = gre[n * N + i];
+            if (gim_out) gim_out[un * N + i] = gim[n * N + i];
+        }
+    }
+    if (info) {
+        info->swept_right_to_left = w.mirrored;
+        info->seconds_setup = w.t_setup;
+        info->seconds_slices = w.t_slices;
+        info->flops_slices = w.flops;
+        if (timer) {
+            info->kernel_seconds = timer->seconds();
+            info->kernel_flops = timer->flops;
+            info->kernel_launches = timer->launches;
+        }
+        info->seconds_total = now() - t0;
     }
 }
 
@@ -523,17 +704,6 @@ void write_smatrix(const Work& w, const Selection& sel, double* S_out) {
     };
     sub_delta(sel.outL, sel.inL, 0, 0);
     sub_delta(sel.outR, sel.inR, nOL, nIL);
-}
-
-void fill_info(rgf_info* info, const Channels& ch, const Selection& sel) {
-    if (!info) return;
-    memset(info, 0, sizeof *info);
-    info->n_transverse = ch.N;
-    info->n_slices = ch.L_slices;
-    info->n_prop_L = ch.L.n_prop;
-    info->n_prop_R = ch.R.n_prop;
-    info->n_out = (int32_t)(sel.outL.size() + sel.outR.size());
-    info->n_in = (int32_t)(sel.inL.size() + sel.inR.size());
 }
 
 void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt, double* S_out,
@@ -793,9 +963,13 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
     });
 }
 
-rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem*, const rgf_selection*, const rgf_options*,
-                                     int32_t, double*, double*, double*, rgf_info*) {
-    return guard(h, [&] { throw Error(RGF_ERR_INVALID_ARGUMENT, "transmission gradient not available"); });
+rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem* p, const rgf_selection* s,
+                                     const rgf_options* opt, int32_t every, double* T_out, double* gre,
+                                     double* gim, rgf_info* info) {
+    return guard(h, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
+        gradient_one(h, p, s, opt, every, T_out, gre, gim, info);
+    });
 }
 
 rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double* A, int32_t block_size) {

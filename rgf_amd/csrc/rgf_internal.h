@@ -65,9 +65,12 @@ struct KernelTimer {
 // ---- kernel launchers (rgf_kernels.hip) -------------------------------------
 // C = alpha * A(MxK) * B(KxN) + beta * C, beta in {0,1}.  Tiles overlapping
 // rows or columns [skip_lo, skip_hi) are left untouched (block Gauss-Jordan).
+// With a workspace (ws, ws_elems elements of T), thin products with a long
+// K are split along K over several work groups and reduced deterministically.
 template <typename T>
 void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C,
-           cd alpha, int beta, int64_t skip_lo = 0, int64_t skip_hi = 0, int use3m = 0);
+           cd alpha, int beta, int64_t skip_lo = 0, int64_t skip_hi = 0, int use3m = 0, T* ws = nullptr,
+           size_t ws_elems = 0);
 
 // In-place inverse of the N x N matrix A by blocked Gauss-Jordan elimination
 // with partial pivoting inside each diagonal block.  ws must hold
@@ -116,6 +119,11 @@ void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const doub
                       const double* eps_im, cd epsL, cd epsR, cd k2, const DevCSR& A0,
                       const double* b0_re, const double* b0_im, int64_t mL, const double* b1_re,
                       const double* b1_im, int64_t slice_stride, int reversed);
+
+// Adjoint contraction for one slice: with u_i = k2 * sum_j psi[i][j] phi[j][i]
+// (psi N x m, phi m x N), dT/dRe(eps_i) = 4 Im(u_i) and dT/dIm(eps_i) = 4 Re(u_i).
+void grad_contract(hipStream_t s, int64_t N, int64_t m, CMat<double> psi, CMat<double> phi, cd k2, double* gre,
+                   double* gim);
 
 // max |x| over a planar FP64 array (result to host, synchronizes the stream).
 double max_abs(hipStream_t s, int64_t count, const double* re, const double* im, double* dev_scratch);

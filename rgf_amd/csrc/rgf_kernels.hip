@@ -22,6 +22,10 @@
 
 namespace rgf {
 
+static int grid_for(int64_t total, int threads = 256, int cap = 16384) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((total + threads - 1) / threads, cap));
+}
+
 // ------------------------------------------------------------------ MFMA wrappers
 template <typename T>
 struct Mfma;
@@ -55,12 +59,27 @@ struct Mfma<float> {
 //
 // USE3M: Re = ArBr - AiBi, Im = (Ar+Ai)(Br+Bi) - ArBr - AiBi (three MFMA
 // chains instead of four; the operand sums are formed once per LDS tile).
+template <typename T>
+struct Vec;  // 16-byte vector of T for the interior-tile global loads
+template <>
+struct Vec<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int W = 4;
+};
+template <>
+struct Vec<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int W = 2;
+};
+
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool USE3M>
 __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
     const T* __restrict__ Are, const T* __restrict__ Aim, int64_t lda, const T* __restrict__ Bre,
     const T* __restrict__ Bim, int64_t ldb, T* Cre, T* Cim, int64_t ldc, int M, int N, int K,
-    double alpha_re, double alpha_im, int beta, int skip_lo, int skip_hi) {
+    double alpha_re, double alpha_im, int beta, int skip_lo, int skip_hi, int vec_ok) {
     using V = typename Mfma<T>::v4;
+    using VT = typename Vec<T>::type;
+    constexpr int VW = Vec<T>::W;
     constexpr int NT = WM * WN * 64;
     constexpr int TM = BM / WM, TN = BN / WN;
     constexpr int FM = TM / 16, FN = TN / 16;
@@ -83,6 +102,10 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
     const int wm = wave / WN, wn = wave % WN;
     const int l16 = lane & 15, l4 = lane >> 4;
 
+    // beta = 1 with alpha = +-1: the accumulators start from C (loads issued
+    // before the K loop) and the sign of alpha is folded into the A operand.
+    const bool accc = beta && alpha_im == 0.0 && (alpha_re == 1.0 || alpha_re == -1.0);
+    const T sgn = accc ? (T)alpha_re : T(1);
     V acc0[FM][FN], acc1[FM][FN], acc2[FM][FN];
 #pragma unroll
     for (int a = 0; a < FM; ++a)
@@ -91,25 +114,122 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
             acc0[a][b] = V{0, 0, 0, 0};
             acc1[a][b] = V{0, 0, 0, 0};
             acc2[a][b] = V{0, 0, 0, 0};
+            if (accc) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(l4, r);
+                    const int gj = tile_n + wn * TN + b * 16 + l16;
+                    if (gi < M && gj < N) {
+                        const int64_t o = (int64_t)gi * ldc + gj;
+                        const T cr = Cre[o], ci = Cim[o];
+                        acc0[a][b][r] = cr;
+                        if (USE3M)
+                            acc2[a][b][r] = cr + ci;  // Im = acc2 - acc0 - acc1 keeps C_im
+                        else
+                            acc1[a][b][r] = ci;
+                    }
+                }
+            }
         }
 
+    // Interior tiles with 16-byte aligned rows load whole vectors without
+    // bounds checks; edge tiles load element by element.
+    const bool interior = vec_ok && tile_m + BM <= M && tile_n + BN <= N && (K % BK) == 0;
     constexpr int AE = BM * BK, BE = BK * BN;
     constexpr int AQ = (AE + NT - 1) / NT, BQ = (BE + NT - 1) / NT;
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    constexpr int AV = AE / VW, BV = BE / VW;
+    constexpr int AQV = (AV + NT - 1) / NT, BQV = (BV + NT - 1) / NT;
+    T pa_r[AQ], pa_i[AQ], pb_r[BQ], pb_i[BQ];
+    VT va_r[AQV], va_i[AQV], vb_r[BQV], vb_i[BQV];
+    const T* Ar0 = Are + (int64_t)tile_m * lda;
+    const T* Ai0 = Aim + (int64_t)tile_m * lda;
+    auto fetch = [&](int k0) {
+        if (interior) {
+#pragma unroll
+            for (int q = 0; q < AQV; ++q) {
+                const int e = tid + q * NT;
+                if (e < AV) {
+                    const int i = e / (BK / VW), kv = (e % (BK / VW)) * VW;
+                    const int o = i * (int)lda + k0 + kv;
+                    va_r[q] = *(const VT*)(Ar0 + o);
+                    va_i[q] = *(const VT*)(Ai0 + o);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BQV; ++q) {
+                const int e = tid + q * NT;
+                if (e < BV) {
+                    const int k = e / (BN / VW), jv = (e % (BN / VW)) * VW;
+                    const int64_t o = (int64_t)(k0 + k) * ldb + tile_n + jv;
+                    vb_r[q] = *(const VT*)(Bre + o);
+                    vb_i[q] = *(const VT*)(Bim + o);
+                }
+            }
+            return;
+        }
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int e = tid + q * NT;
+            const int i = e / BK, k = e % BK;
+            const int gi = tile_m + i, gk = k0 + k;
+            pa_r[q] = 0;
+            pa_i[q] = 0;
+            if (e < AE && gi < M && gk < K) {
+                pa_r[q] = Are[(int64_t)gi * lda + gk];
+                pa_i[q] = Aim[(int64_t)gi * lda + gk];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int e = tid + q * NT;
+            const int k = e / BN, j = e % BN;
+            const int gk = k0 + k, gj = tile_n + j;
+            pb_r[q] = 0;
+            pb_i[q] = 0;
+            if (e < BE && gk < K && gj < N) {
+                pb_r[q] = Bre[(int64_t)gk * ldb + gj];
+                pb_i[q] = Bim[(int64_t)gk * ldb + gj];
+            }
+        }
+    };
+    auto stash = [&]() {
+        if (interior) {
+#pragma unroll
+            for (int q = 0; q < AQV; ++q) {
+                const int e = tid + q * NT;
+                if (e < AV) {
+                    const int i = e / (BK / VW), kv = (e % (BK / VW)) * VW;
+#pragma unroll
+                    for (int u = 0; u < VW; ++u) {
+                        sA[0][i][kv + u] = va_r[q][u];
+                        sA[1][i][kv + u] = va_i[q][u];
+                        if (USE3M) sA[NP - 1][i][kv + u] = va_r[q][u] + va_i[q][u];
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BQV; ++q) {
+                const int e = tid + q * NT;
+                if (e < BV) {
+                    const int k = e / (BN / VW), jv = (e % (BN / VW)) * VW;
+#pragma unroll
+                    for (int u = 0; u < VW; ++u) {
+                        sB[0][k][jv + u] = vb_r[q][u];
+                        sB[1][k][jv + u] = vb_i[q][u];
+                        if (USE3M) sB[NP - 1][k][jv + u] = vb_r[q][u] + vb_i[q][u];
+                    }
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < AQ; ++q) {
             const int e = tid + q * NT;
             if (e < AE) {
                 const int i = e / BK, k = e % BK;
-                const int gi = tile_m + i, gk = k0 + k;
-                T vr = 0, vi = 0;
-                if (gi < M && gk < K) {
-                    vr = Are[(int64_t)gi * lda + gk];
-                    vi = Aim[(int64_t)gi * lda + gk];
-                }
-                sA[0][i][k] = vr;
-                sA[1][i][k] = vi;
-                if (USE3M) sA[NP - 1][i][k] = vr + vi;
+                sA[0][i][k] = pa_r[q];
+                sA[1][i][k] = pa_i[q];
+                if (USE3M) sA[NP - 1][i][k] = pa_r[q] + pa_i[q];
             }
         }
 #pragma unroll
@@ -117,26 +237,27 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
             const int e = tid + q * NT;
             if (e < BE) {
                 const int k = e / BN, j = e % BN;
-                const int gk = k0 + k, gj = tile_n + j;
-                T vr = 0, vi = 0;
-                if (gk < K && gj < N) {
-                    vr = Bre[(int64_t)gk * ldb + gj];
-                    vi = Bim[(int64_t)gk * ldb + gj];
-                }
-                sB[0][k][j] = vr;
-                sB[1][k][j] = vi;
-                if (USE3M) sB[NP - 1][k][j] = vr + vi;
+                sB[0][k][j] = pb_r[q];
+                sB[1][k][j] = pb_i[q];
+                if (USE3M) sB[NP - 1][k][j] = pb_r[q] + pb_i[q];
             }
         }
+    };
+
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        stash();
         __syncthreads();
+        if (k0 + BK < K) fetch(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            T ar[FM], ai[FM], br[FN], bi[FN];
+            T ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
 #pragma unroll
             for (int a = 0; a < FM; ++a) {
                 const int r = wm * TM + a * 16 + l16;
-                ar[a] = sA[0][r][kk + l4];
-                ai[a] = sA[1][r][kk + l4];
+                ar[a] = sgn * sA[0][r][kk + l4];
+                ai[a] = sgn * sA[1][r][kk + l4];
+                nai[a] = -ai[a];
             }
 #pragma unroll
             for (int b = 0; b < FN; ++b) {
@@ -144,28 +265,39 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
                 br[b] = sB[0][kk + l4][c];
                 bi[b] = sB[1][kk + l4][c];
             }
+            // independent MFMA sweeps: a chain's dependent MFMAs are
+            // FM*FN (or 2*FM*FN) instructions apart
             if (USE3M) {
                 T as[FM], bs[FN];
 #pragma unroll
-                for (int a = 0; a < FM; ++a) as[a] = sA[NP - 1][wm * TM + a * 16 + l16][kk + l4];
+                for (int a = 0; a < FM; ++a) as[a] = sgn * sA[NP - 1][wm * TM + a * 16 + l16][kk + l4];
 #pragma unroll
                 for (int b = 0; b < FN; ++b) bs[b] = sB[NP - 1][kk + l4][wn * TN + b * 16 + l16];
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
-                    for (int b = 0; b < FN; ++b) {
-                        acc0[a][b] = Mfma<T>::mma(ar[a], br[b], acc0[a][b]);  // ArBr
-                        acc1[a][b] = Mfma<T>::mma(ai[a], bi[b], acc1[a][b]);  // AiBi
-                        acc2[a][b] = Mfma<T>::mma(as[a], bs[b], acc2[a][b]);  // (Ar+Ai)(Br+Bi)
-                    }
+                    for (int b = 0; b < FN; ++b) acc0[a][b] = Mfma<T>::mma(ar[a], br[b], acc0[a][b]);
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) acc1[a][b] = Mfma<T>::mma(ai[a], bi[b], acc1[a][b]);
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) acc2[a][b] = Mfma<T>::mma(as[a], bs[b], acc2[a][b]);
             } else {
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
                     for (int b = 0; b < FN; ++b) {
                         acc0[a][b] = Mfma<T>::mma(ar[a], br[b], acc0[a][b]);
-                        acc0[a][b] = Mfma<T>::mma(-ai[a], bi[b], acc0[a][b]);
                         acc1[a][b] = Mfma<T>::mma(ar[a], bi[b], acc1[a][b]);
+                    }
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) {
+                        acc0[a][b] = Mfma<T>::mma(nai[a], bi[b], acc0[a][b]);
                         acc1[a][b] = Mfma<T>::mma(ai[a], br[b], acc1[a][b]);
                     }
             }
@@ -173,7 +305,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
         __syncthreads();
     }
 
-    // epilogue: C = alpha * acc (+ C)
+    // epilogue
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -184,23 +316,28 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
                 const int gj = tile_n + wn * TN + b * 16 + l16;
                 const bool masked = (gi >= skip_lo && gi < skip_hi) || (gj >= skip_lo && gj < skip_hi);
                 if (gi < M && gj < N && !masked) {
-                    double pr, pi;
+                    T pr, pi;
                     if (USE3M) {
-                        pr = (double)acc0[a][b][r] - (double)acc1[a][b][r];
-                        pi = (double)acc2[a][b][r] - (double)acc0[a][b][r] - (double)acc1[a][b][r];
+                        pr = acc0[a][b][r] - acc1[a][b][r];
+                        pi = acc2[a][b][r] - acc0[a][b][r] - acc1[a][b][r];
                     } else {
                         pr = acc0[a][b][r];
                         pi = acc1[a][b][r];
                     }
-                    double vr = alpha_re * pr - alpha_im * pi;
-                    double vi = alpha_re * pi + alpha_im * pr;
                     const int64_t o = (int64_t)gi * ldc + gj;
-                    if (beta) {
-                        vr += (double)Cre[o];
-                        vi += (double)Cim[o];
+                    if (accc) {
+                        Cre[o] = pr;
+                        Cim[o] = pi;
+                    } else {
+                        double vr = alpha_re * (double)pr - alpha_im * (double)pi;
+                        double vi = alpha_re * (double)pi + alpha_im * (double)pr;
+                        if (beta) {
+                            vr += (double)Cre[o];
+                            vi += (double)Cim[o];
+                        }
+                        Cre[o] = (T)vr;
+                        Cim[o] = (T)vi;
                     }
-                    Cre[o] = (T)vr;
-                    Cim[o] = (T)vi;
                 }
             }
 }
@@ -211,7 +348,7 @@ struct GemmCfg;
 // FP32: same shapes with a deeper K chunk.
 template <>
 struct GemmCfg<double> {
-    static constexpr int BK_SMALL = 16, BK_BIG = 16;
+    static constexpr int BK_SMALL = 16, BK_BIG = 8;
 };
 template <>
 struct GemmCfg<float> {
@@ -223,149 +360,362 @@ static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T>
                          cd alpha, int beta, int64_t skip_lo, int64_t skip_hi, int use3m) {
     dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
     dim3 block(WM * WN * 64);
+    // vector loads need 16-byte aligned rows in A and B
+    const int VW = 16 / (int)sizeof(T);
+    const int vec_ok = (A.ld % VW == 0) && (B.ld % VW == 0) && ((uintptr_t)A.re % 16 == 0) &&
+                       ((uintptr_t)A.im % 16 == 0) && ((uintptr_t)B.re % 16 == 0) && ((uintptr_t)B.im % 16 == 0);
     if (use3m)
         hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, true>), grid, block, 0, s, A.re, A.im, A.ld,
                            B.re, B.im, B.ld, C.re, C.im, C.ld, (int)M, (int)N, (int)K, alpha.real(),
-                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi);
+                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi, vec_ok);
     else
         hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, false>), grid, block, 0, s, A.re, A.im, A.ld,
                            B.re, B.im, B.ld, C.re, C.im, C.ld, (int)M, (int)N, (int)K, alpha.real(),
-                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi);
+                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi, vec_ok);
+}
+
+// C = alpha * sum_z P_z + beta * C over the split-K partial products
+template <typename T>
+__global__ void splitk_reduce_kernel(int64_t M, int64_t N, int nz, const T* P, int64_t pstride, T* Cre, T* Cim,
+                                     int64_t ldc, double ar, double ai, int beta) {
+    const int64_t total = M * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        double sr = 0, si = 0;
+        for (int z = 0; z < nz; ++z) {
+            sr += (double)P[z * pstride + e];
+            si += (double)P[z * pstride + total + e];
+        }
+        const int64_t i = e / N, j = e % N;
+        const int64_t o = i * ldc + j;
+        double vr = ar * sr - ai * si, vi = ar * si + ai * sr;
+        if (beta) {
+            vr += (double)Cre[o];
+            vi += (double)Cim[o];
+        }
+        Cre[o] = (T)vr;
+        Cim[o] = (T)vi;
+    }
 }
 
 template <typename T>
 void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
-           int beta, int64_t skip_lo, int64_t skip_hi, int use3m) {
+           int beta, int64_t skip_lo, int64_t skip_hi, int use3m, T* ws, size_t ws_elems) {
     if (M <= 0 || N <= 0) return;
+    // split-K for thin products with a long inner dimension
+    if (ws && skip_hi <= skip_lo && K >= 1024) {
+        const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
+        if (tiles < 512) {
+            const int64_t per = 2 * M * N;  // elements of one partial (two planes)
+            int64_t nz = std::min<int64_t>((1024 + tiles - 1) / tiles, K / 256);
+            nz = std::min<int64_t>(nz, (int64_t)(ws_elems / per));
+            if (nz >= 2) {
+                constexpr int BK = GemmCfg<T>::BK_SMALL;
+                const int64_t kc = ((K + nz - 1) / nz + BK - 1) / BK * BK;
+                nz = (K + kc - 1) / kc;
+                for (int64_t z = 0; z < nz; ++z) {
+                    const int64_t k0 = z * kc, kl = std::min<int64_t>(kc, K - k0);
+                    CMat<T> Az(A.re + k0, A.im + k0, A.ld);
+                    CMat<T> Bz(B.re + k0 * B.ld, B.im + k0 * B.ld, B.ld);
+                    CMat<T> Pz(ws + z * per, ws + z * per + M * N, N);
+                    launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, kl, Az, Bz, Pz, cd(1, 0), 0, 0, 0,
+                                                                        use3m);
+                }
+                hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
+                                   (const T*)ws, per, C.re, C.im, C.ld, alpha.real(), alpha.imag(), beta);
+                RGF_HIP(hipGetLastError());
+                return;
+            }
+        }
+    }
     if (K <= 0) {
         if (!beta) fill_zero<T>(s, M, N, C);
         return;
     }
     if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX) throw Error(RGF_ERR_INVALID_ARGUMENT, "GEMM too large");
     const int64_t tiles_big = ((M + 127) / 128) * ((N + 127) / 128);
+    const int64_t tiles_mid = ((M + 63) / 64) * ((N + 63) / 64);
     if (tiles_big >= 1024)
         launch_cgemm<T, 128, 128, GemmCfg<T>::BK_BIG, 2, 4>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
-    else
+    else if (tiles_mid >= 512)
         launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
+    else
+        launch_cgemm<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
     RGF_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ K2: diagonal-block inverse
 // One work group inverts the nb x nb diagonal block D = A[p0:p0+nb, p0:p0+nb]
-// in LDS by Gauss-Jordan elimination with partial (row) pivoting inside the
-// block, and writes D^-1 back to A and to Dout (ld = ldd).  The column
-// interchanges that undo the row pivoting are folded into the write-back.
-template <typename T, int BMAX, int NT>
-__global__ __launch_bounds__(NT) void gj_diag_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb, T* Dre,
-                                                     T* Dim, int64_t ldd, int* status) {
-    __shared__ T sr[BMAX][BMAX + 1];
-    __shared__ T si[BMAX][BMAX + 1];
-    __shared__ T rowr[BMAX], rowi[BMAX], colr[BMAX], coli[BMAX];
-    __shared__ int pivs[BMAX];
-    __shared__ int colsrc[BMAX];
-    __shared__ int piv_s;
-    __shared__ T pvr_s, pvi_s;
+// (nb <= NB, padded with the identity to NB) by Gauss-Jordan elimination with
+// row interchanges confined to the block, writes D^-1 back to A and to Dout
+// (ld = ldd), and folds the column interchanges that undo the row pivoting
+// into the write-back.
+// The nb x nb block (padded with the identity to
+// NB) lives in LDS.  For each 16-column panel, wavefront 0 keeps the panel
+// (all NB rows) in registers and runs 16 Gauss-Jordan steps with partial
+// pivoting over all remaining rows, broadcasting the pivot row with
+// readlane (no barriers inside a panel).  The other columns are then
+// brought up to date by all wavefronts: the 16 row interchanges, then the
+// rank-16 block sweep  M[:, J] = (rows P ? 0 : M[:, J]) + M'[:, P] * M_old[P, J]
+// on MFMA, where M'[:, P] are the panel's new values.
+template <typename T>
+__device__ __forceinline__ T lane_bcast(T v, int l);
+template <>
+__device__ __forceinline__ float lane_bcast<float>(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+template <>
+__device__ __forceinline__ double lane_bcast<double>(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 64-bit maximum over a wavefront (DPP-free butterfly through ds_swizzle is
+// not needed: the key reductions happen once per pivot or once per panel).
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off);
+        if (o > v) v = o;
+    }
+    return v;
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(512) void gj_diag_blk_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb, T* Dre,
+                                                          T* Dim, int64_t ldd, int* status) {
+    using V = typename Mfma<T>::v4;
+    constexpr int NT = 512;
+    constexpr int PW = 16;        // panel width
+    constexpr int RPL = NB / 64;  // rows per lane in wavefront 0
+    constexpr int LDP = NB + 1;
+    constexpr int NBLK = NB / 16;
+    static_assert(NB % 64 == 0, "NB must be a multiple of 64");
+    __shared__ T sr[NB][LDP], si[NB][LDP];
+    __shared__ T orr[PW][NB], ori[PW][NB];  // M_old[P, :] for the block sweep
+    __shared__ int pivs[NB];
+    __shared__ int bad_s;
     const int tid = threadIdx.x;
-    for (int e = tid; e < nb * nb; e += NT) {
-        const int i = e / nb, j = e % nb;
-        sr[i][j] = Are[(int64_t)(p0 + i) * lda + p0 + j];
-        si[i][j] = Aim[(int64_t)(p0 + i) * lda + p0 + j];
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    if (tid == 0) bad_s = 0;
+    for (int e = tid; e < NB * NB; e += NT) {
+        const int i = e / NB, j = e % NB;
+        T vr = (i == j) ? T(1) : T(0), vi = 0;
+        if (i < nb && j < nb) {
+            vr = Are[(int64_t)(p0 + i) * lda + p0 + j];
+            vi = Aim[(int64_t)(p0 + i) * lda + p0 + j];
+        }
+        sr[i][j] = vr;
+        si[i][j] = vi;
     }
     __syncthreads();
-    int bad = 0;
-    for (int k = 0; k < nb; ++k) {
-        if (tid < 64) {  // pivot search by wavefront 0
-            T best = -1;
-            int bi = k;
-            for (int r = k + tid; r < nb; r += 64) {
-                const T m = sr[r][k] * sr[r][k] + si[r][k] * si[r][k];
-                if (m > best) {
-                    best = m;
-                    bi = r;
+    for (int q0 = 0; q0 < NB; q0 += PW) {
+        // ---- phase 1: wavefront 0 factors the panel [q0, q0+16).  First
+        // without row interchanges (the slice matrices are close to diagonally
+        // dominant); if the panel grows by more than 1e3 the attempt is
+        // discarded and the panel is factored again with partial pivoting.
+        if (wave == 0) {
+            T xr[RPL][PW], xi[RPL][PW];
+            int bad = 0;
+            float m0 = 0.f;
+#pragma unroll 1
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                const bool pivoting = attempt == 1;
+#pragma unroll
+                for (int s = 0; s < RPL; ++s)
+#pragma unroll
+                    for (int c = 0; c < PW; ++c) {
+                        xr[s][c] = sr[lane + 64 * s][q0 + c];
+                        xi[s][c] = si[lane + 64 * s][q0 + c];
+                        if (attempt == 0) m0 = fmaxf(m0, (float)(xr[s][c] * xr[s][c] + xi[s][c] * xi[s][c]));
+                    }
+                bad = 0;
+#pragma unroll
+                for (int k = 0; k < PW; ++k) {
+                    const int kc = q0 + k;
+                    int piv = kc;
+                    if (pivoting) {
+                        // largest |x| among rows >= kc, lowest row on ties
+                        unsigned long long key = 0;
+#pragma unroll
+                        for (int s = 0; s < RPL; ++s) {
+                            const int row = lane + 64 * s;
+                            const float m = (float)(xr[s][k] * xr[s][k] + xi[s][k] * xi[s][k]);
+                            const unsigned long long kk = ((unsigned long long)__float_as_uint(m) << 32) |
+                                                          (unsigned long long)(0xffffffffu - (unsigned)row);
+                            if (row >= kc && kk > key) key = kk;
+                        }
+                        key = wave_max_u64(key);
+                        piv = (int)(0xffffffffu - (unsigned)(key & 0xffffffffu));
+                    }
+                    const int la = kc & 63, sa = kc >> 6, lb = piv & 63, sb = piv >> 6;
+                    // pivot value (row piv, column k) and its inverse
+                    T vk_r = xr[0][k], vk_i = xi[0][k];
+#pragma unroll
+                    for (int s = 1; s < RPL; ++s)
+                        if (sb == s) {
+                            vk_r = xr[s][k];
+                            vk_i = xi[s][k];
+                        }
+                    const T pkr = lane_bcast(vk_r, lb), pki = lane_bcast(vk_i, lb);
+                    const T den = pkr * pkr + pki * pki;
+                    if (!(den > 0) || !isfinite(den)) bad = 1;
+                    const T ivr = pkr / den, ivi = -pki / den;
+                    // scaled pivot row; with pivoting, row piv receives the old row kc
+                    T rr[PW], ri[PW];
+#pragma unroll
+                    for (int c = 0; c < PW; ++c) {
+                        T vb_r = xr[0][c], vb_i = xi[0][c], va_r = xr[0][c], va_i = xi[0][c];
+#pragma unroll
+                        for (int s = 1; s < RPL; ++s) {
+                            if (sb == s) {
+                                vb_r = xr[s][c];
+                                vb_i = xi[s][c];
+                            }
+                            if (sa == s) {
+                                va_r = xr[s][c];
+                                va_i = xi[s][c];
+                            }
+                        }
+                        const T br = lane_bcast(vb_r, lb), bi2 = lane_bcast(vb_i, lb);
+                        rr[c] = (c == k) ? ivr : br * ivr - bi2 * ivi;
+                        ri[c] = (c == k) ? ivi : br * ivi + bi2 * ivr;
+                        if (pivoting && piv != kc) {
+                            const T ar = lane_bcast(va_r, la), ai = lane_bcast(va_i, la);
+#pragma unroll
+                            for (int s = 0; s < RPL; ++s)
+                                if (lane + 64 * s == piv) {
+                                    xr[s][c] = ar;
+                                    xi[s][c] = ai;
+                                }
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < RPL; ++s) {
+                        const int row = lane + 64 * s;
+                        if (row == kc) {
+#pragma unroll
+                            for (int c = 0; c < PW; ++c) {
+                                xr[s][c] = rr[c];
+                                xi[s][c] = ri[c];
+                            }
+                        } else {
+                            const T fr = xr[s][k], fi = xi[s][k];
+#pragma unroll
+                            for (int c = 0; c < PW; ++c) {
+                                const T br = (c == k) ? T(0) : xr[s][c];
+                                const T bi2 = (c == k) ? T(0) : xi[s][c];
+                                xr[s][c] = br - (fr * rr[c] - fi * ri[c]);
+                                xi[s][c] = bi2 - (fr * ri[c] + fi * rr[c]);
+                            }
+                        }
+                    }
+                    if (lane == 0) pivs[kc] = piv;
+                }
+                if (pivoting) break;
+                // growth check of the unpivoted attempt (one reduction per panel)
+                float m1 = 0.f;
+#pragma unroll
+                for (int s = 0; s < RPL; ++s)
+#pragma unroll
+                    for (int c = 0; c < PW; ++c)
+                        m1 = fmaxf(m1, (float)(xr[s][c] * xr[s][c] + xi[s][c] * xi[s][c]));
+                const unsigned long long g =
+                    wave_max_u64(((unsigned long long)__float_as_uint(m1) << 32) | __float_as_uint(m0));
+                const float gm1 = __uint_as_float((unsigned)(g >> 32));
+                float gm0 = __uint_as_float((unsigned)(wave_max_u64(__float_as_uint(m0)) & 0xffffffffu));
+                const bool ok = !bad && isfinite(gm1) && gm1 <= 1e6f * fmaxf(gm0, 1e-30f);
+                if (ok) break;
+            }
+#pragma unroll
+            for (int s = 0; s < RPL; ++s)
+#pragma unroll
+                for (int c = 0; c < PW; ++c) {
+                    sr[lane + 64 * s][q0 + c] = xr[s][c];
+                    si[lane + 64 * s][q0 + c] = xi[s][c];
+                }
+            if (bad && lane == 0) bad_s = 1;
+        }
+        __syncthreads();
+        // ---- phase 2a: row interchanges on the other columns; keep M_old[P, :]
+        for (int j = tid; j < NB; j += NT) {
+            if (j >= q0 && j < q0 + PW) continue;
+            for (int k = 0; k < PW; ++k) {
+                const int a = q0 + k, b = pivs[a];
+                if (b != a) {
+                    const T tr = sr[a][j], ti = si[a][j];
+                    sr[a][j] = sr[b][j];
+                    si[a][j] = si[b][j];
+                    sr[b][j] = tr;
+                    si[b][j] = ti;
                 }
             }
 #pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                const T ob = __shfl_xor(best, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ob > best || (ob == best && oi < bi)) {
-                    best = ob;
-                    bi = oi;
-                }
-            }
-            if (tid == 0) {
-                piv_s = bi;
-                pvr_s = sr[bi][k];
-                pvi_s = si[bi][k];
-                pivs[k] = bi;
+            for (int k = 0; k < PW; ++k) {
+                orr[k][j] = sr[q0 + k][j];
+                ori[k][j] = si[q0 + k][j];
             }
         }
         __syncthreads();
-        const int piv = piv_s;
-        const T pr = pvr_s, pim = pvi_s;
-        const T den = pr * pr + pim * pim;
-        const T ivr = pr / den, ivi = -pim / den;  // 1 / pivot
-        if (!(den > 0) || !isfinite(den)) bad = 1;
-        // new row k = old row piv scaled; old row k moves to row piv; save column k
-        for (int j = tid; j < nb; j += NT) {
-            const T xr = sr[piv][j], xi = si[piv][j];
-            if (j == k) {
-                rowr[j] = ivr;
-                rowi[j] = ivi;
-            } else {
-                rowr[j] = xr * ivr - xi * ivi;
-                rowi[j] = xr * ivi + xi * ivr;
+        // ---- phase 2b: rank-16 block sweep of the other columns on MFMA
+        for (int t = wave; t < NBLK * NBLK; t += NT / 64) {
+            const int bi = t / NBLK, bj = t % NBLK;
+            if (bj * 16 == q0) continue;
+            const int l16 = lane & 15, l4 = lane >> 4;
+            V a0, a1;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = bi * 16 + Mfma<T>::row(l4, r), j = bj * 16 + l16;
+                const bool inP = (i >= q0 && i < q0 + PW);
+                a0[r] = inP ? T(0) : sr[i][j];
+                a1[r] = inP ? T(0) : si[i][j];
             }
-            if (piv != k) {
-                sr[piv][j] = sr[k][j];
-                si[piv][j] = si[k][j];
+#pragma unroll
+            for (int kk = 0; kk < PW; kk += 4) {
+                const T ar = sr[bi * 16 + l16][q0 + kk + l4], ai = si[bi * 16 + l16][q0 + kk + l4];
+                const T br = orr[kk + l4][bj * 16 + l16], bi2 = ori[kk + l4][bj * 16 + l16];
+                a0 = Mfma<T>::mma(ar, br, a0);
+                a1 = Mfma<T>::mma(ar, bi2, a1);
+                a0 = Mfma<T>::mma(-ai, bi2, a0);
+                a1 = Mfma<T>::mma(ai, br, a1);
             }
-        }
-        for (int r = tid; r < nb; r += NT) {
-            const T cr = (r == piv) ? sr[k][k] : sr[r][k];
-            const T ci = (r == piv) ? si[k][k] : si[r][k];
-            colr[r] = cr;
-            coli[r] = ci;
-        }
-        __syncthreads();
-        for (int e = tid; e < nb * nb; e += NT) {
-            const int r = e / nb, j = e % nb;
-            if (r == k) {
-                sr[r][j] = rowr[j];
-                si[r][j] = rowi[j];
-            } else if (j == k) {
-                const T cr = colr[r], ci = coli[r];
-                sr[r][j] = -(cr * ivr - ci * ivi);
-                si[r][j] = -(cr * ivi + ci * ivr);
-            } else {
-                const T cr = colr[r], ci = coli[r];
-                const T xr = rowr[j], xi = rowi[j];
-                sr[r][j] -= cr * xr - ci * xi;
-                si[r][j] -= cr * xi + ci * xr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = bi * 16 + Mfma<T>::row(l4, r), j = bj * 16 + l16;
+                sr[i][j] = a0[r];
+                si[i][j] = a1[r];
             }
         }
         __syncthreads();
     }
+    // column un-permutation: out[:, j] = X[:, src[j]]; pivs becomes the
+    // destination column of each X column
+    __shared__ int srcs[NB];
     if (tid == 0) {
-        for (int j = 0; j < nb; ++j) colsrc[j] = j;
-        for (int k = nb - 1; k >= 0; --k) {
+        for (int j = 0; j < NB; ++j) srcs[j] = j;
+        for (int k = NB - 1; k >= 0; --k) {
             const int p = pivs[k];
-            const int t = colsrc[k];
-            colsrc[k] = colsrc[p];
-            colsrc[p] = t;
+            const int t2 = srcs[k];
+            srcs[k] = srcs[p];
+            srcs[p] = t2;
         }
+        for (int j = 0; j < NB; ++j) pivs[srcs[j]] = j;
     }
     __syncthreads();
-    for (int e = tid; e < nb * nb; e += NT) {
-        const int i = e / nb, j = e % nb;
-        const int js = colsrc[j];
-        const T vr = sr[i][js], vi = si[i][js];
-        Are[(int64_t)(p0 + i) * lda + p0 + j] = vr;
-        Aim[(int64_t)(p0 + i) * lda + p0 + j] = vi;
-        Dre[(int64_t)i * ldd + j] = vr;
-        Dim[(int64_t)i * ldd + j] = vi;
+    for (int e = tid; e < NB * NB; e += NT) {
+        const int i = e / NB, c = e % NB;
+        const int j = pivs[c];
+        if (i < nb && j < nb) {
+            Are[(int64_t)(p0 + i) * lda + p0 + j] = sr[i][c];
+            Aim[(int64_t)(p0 + i) * lda + p0 + j] = si[i][c];
+            Dre[(int64_t)i * ldd + j] = sr[i][c];
+            Dim[(int64_t)i * ldd + j] = si[i][c];
+        }
     }
-    if (bad && tid == 0) atomicOr(status, 1);
+    if (tid == 0 && bad_s) atomicOr(status, 1);
 }
 
 // Copy the new row panel R (nb x N, rows of Rbuf) into A[p0:p0+nb, :] and the
@@ -397,7 +747,7 @@ __global__ void gj_writeback_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int 
 
 template <typename T>
 static constexpr int diag_bmax() {
-    return sizeof(T) == 8 ? 64 : 128;
+    return sizeof(T) == 8 ? 64 : 128;  // the diagonal block lives in LDS
 }
 
 template <typename T>
@@ -454,9 +804,18 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
     CMat<T> R(Rre, Rim, N), C(Cre, Cim, b), D(Dre, Dim, b);
     for (int64_t p0 = 0; p0 < N; p0 += b) {
         const int nb = (int)std::min<int64_t>(b, N - p0);
-        constexpr int BMAX = diag_bmax<T>();
-        hipLaunchKernelGGL((gj_diag_kernel<T, BMAX, 512>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb,
-                           D.re, D.im, D.ld, dev_status);
+        if constexpr (diag_bmax<T>() >= 128) {
+            if (nb > 64) {
+                hipLaunchKernelGGL((gj_diag_blk_kernel<T, 128>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0,
+                                   nb, D.re, D.im, D.ld, dev_status);
+            } else {
+                hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0,
+                                   nb, D.re, D.im, D.ld, dev_status);
+            }
+        } else {
+            hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb,
+                               D.re, D.im, D.ld, dev_status);
+        }
         if (nb == N) break;  // single block: done
         // R = D^-1 * A[P, :]
         cgemm<T>(s, nb, N, nb, D, A.offset(p0, 0), R, cd(1, 0), 0, 0, 0, use3m);
@@ -512,9 +871,6 @@ __global__ void csr_add_kernel(int32_t n, const int32_t* rowptr, const int32_t* 
     }
 }
 
-static int grid_for(int64_t total, int threads = 256, int cap = 16384) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>((total + threads - 1) / threads, cap));
-}
 
 template <typename T>
 void assemble_slice(hipStream_t s, int64_t N, CMat<T> M, const double* eps_re, const double* eps_im,
@@ -714,6 +1070,31 @@ void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const doub
     RGF_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ adjoint contraction
+__global__ void grad_contract_kernel(int64_t N, int64_t m, const double* pr, const double* pim, int64_t ldp,
+                                     const double* fr, const double* fi, int64_t ldf, double k2r, double k2i,
+                                     double* gre, double* gim) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double ar = 0, ai = 0;
+    for (int64_t j = 0; j < m; ++j) {
+        const double xr = pr[i * ldp + j], xi = pim[i * ldp + j];
+        const double yr = fr[j * ldf + i], yi = fi[j * ldf + i];
+        ar += xr * yr - xi * yi;
+        ai += xr * yi + xi * yr;
+    }
+    const double ur = k2r * ar - k2i * ai, ui = k2r * ai + k2i * ar;
+    gre[i] = 4.0 * ui;
+    gim[i] = 4.0 * ur;
+}
+
+void grad_contract(hipStream_t s, int64_t N, int64_t m, CMat<double> psi, CMat<double> phi, cd k2, double* gre,
+                   double* gim) {
+    hipLaunchKernelGGL(grad_contract_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, m, psi.re, psi.im,
+                       psi.ld, phi.re, phi.im, phi.ld, k2.real(), k2.imag(), gre, gim);
+    RGF_HIP(hipGetLastError());
+}
+
 __global__ void max_abs_kernel(int64_t count, const double* re, const double* im, unsigned long long* out) {
     double m = 0.0;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
@@ -746,7 +1127,7 @@ double max_abs(hipStream_t s, int64_t count, const double* re, const double* im,
 // ------------------------------------------------------------------ instantiations
 #define RGF_INST(T)                                                                                          \
     template void cgemm<T>(hipStream_t, int64_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>, cd, int, int64_t, \
-                           int64_t, int);                                                                     \
+                           int64_t, int, T*, size_t);                                                         \
     template size_t inverse_workspace_elems<T>(int64_t, int);                                                 \
     template void invert_inplace<T>(hipStream_t, int64_t, CMat<T>, int, T*, int*, int, KernelTimer*);         \
     template void assemble_slice<T>(hipStream_t, int64_t, CMat<T>, const double*, const double*, cd, cd,      \

@@ -1,0 +1,312 @@
+/*
+ * MATLAB/Octave MEX gateway over librgf (include/rgf.h).
+ *
+ *   [smatrix, info] = cal_smatrix_RGF_mex(syst, out, in [, opts])
+ *   [T, dT_deps]     = cal_smatrix_RGF_mex('gradient', syst, out, in [, opts])
+ *
+ * It reproduces the calling convention of cal_smatrix_RGF.m:1 and
+ * cal_smatrix_RGF_3D.m:1 of chiaweihsu/RGF: `syst` is the same struct
+ * (epsilon, epsilon_L, epsilon_R, k0dx and yBC (2D) or xyBC (3D)); `out`
+ * and `in` are cells {'L','R'} or structs with ind_out_L/ind_out_R and
+ * ind_in_L/ind_in_R.  A 3D epsilon (or a syst with xyBC) selects the 3D
+ * solver.  `opts` may hold precision ('fp64' | 'fp32_refined'), refine_tol,
+ * max_refine_iter and checkpoint_every.  The `channels` output of the
+ * reference is produced by the thin wrappers cal_smatrix_RGF.m /
+ * cal_smatrix_RGF_3D.m next to this file, which call setup_channels.
+ *
+ * The gateway only marshals: every check and every number comes from
+ * librgf.  Device resources live in one handle created on first use and
+ * released by mexAtExit.  librgf frees its per-call state before it returns
+ * a status, so raising an error through mexErrMsgIdAndTxt (which does not
+ * return) leaves the handle valid; host buffers are mxMalloc'ed and are
+ * reclaimed by MATLAB on error.
+ *
+ * Build (MATLAB R2018a+ interleaved complex API, or Octave's mkoctfile):
+ *   mex -R2018a -I../../include cal_smatrix_RGF_mex.c -L.. -lrgf
+ *   mkoctfile --mex -I../../include cal_smatrix_RGF_mex.c -L.. -lrgf
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mex.h"
+#include "rgf.h"
+
+static rgf_handle g_handle = NULL;
+
+static void release_handle(void) {
+    if (g_handle) {
+        rgf_destroy(g_handle);
+        g_handle = NULL;
+    }
+}
+
+static void fail(const char* id, const char* msg) { mexErrMsgIdAndTxt(id, "%s", msg); }
+
+static void check(rgf_status st, rgf_handle h) {
+    if (st != RGF_OK) {
+        const char* m = rgf_last_error(h);
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s", m && *m ? m : rgf_status_string(st));
+        fail("RGF:error", buf);
+    }
+}
+
+static double scalar_part(const mxArray* a, int imag_part, const char* name) {
+    if (!a || !mxIsNumeric(a) || mxGetNumberOfElements(a) != 1) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "%s must be a numeric scalar", name);
+        fail("RGF:input", buf);
+    }
+    if (mxIsComplex(a)) {
+        mxComplexDouble* z = mxGetComplexDoubles(a);
+        return imag_part ? z[0].imag : z[0].real;
+    }
+    return imag_part ? 0.0 : mxGetScalar(a);
+}
+
+static int streq_ci(const char* a, const char* b) {
+    for (; *a && *b; ++a, ++b) {
+        char x = (*a >= 'A' && *a <= 'Z') ? *a + 32 : *a;
+        char y = (*b >= 'A' && *b <= 'Z') ? *b + 32 : *b;
+        if (x != y) return 0;
+    }
+    return *a == *b;
+}
+
+/* setup_channels.m:102-159: names are case-insensitive; numbers are Bloch phases */
+static void parse_bc(const mxArray* bc, int ndim, rgf_problem* p) {
+    if (mxIsChar(bc)) {
+        char s[64];
+        mxGetString(bc, s, sizeof s);
+        int code = -1;
+        double ka = 0.0;
+        if (streq_ci(s, "periodic")) code = RGF_BC_BLOCH;
+        else if (streq_ci(s, "dirichlet") || streq_ci(s, "pec")) code = RGF_BC_DIRICHLET;
+        else if (ndim == 2 && (streq_ci(s, "neumann") || streq_ci(s, "pmc"))) code = RGF_BC_NEUMANN;
+        else if (ndim == 2 && (streq_ci(s, "dirichletneumann") || streq_ci(s, "pecpmc"))) code = RGF_BC_DIRICHLET_NEUMANN;
+        else if (ndim == 2 && (streq_ci(s, "neumanndirichlet") || streq_ci(s, "pmcpec"))) code = RGF_BC_NEUMANN_DIRICHLET;
+        else if (streq_ci(s, "bloch"))
+            fail("RGF:input", "To use Bloch periodic boundary condition, set BC to k*a where k is the Bloch wave number and a is the periodicity");
+        if (code < 0) {
+            char buf[128];
+            snprintf(buf, sizeof buf, "Boundary condition BC = %s is not an eligible option", s);
+            fail("RGF:input", buf);
+        }
+        p->bc[0] = p->bc[1] = code;
+        p->bloch_ka[0] = p->bloch_ka[1] = ka;
+        return;
+    }
+    if (!mxIsNumeric(bc) || mxIsComplex(bc)) fail("RGF:input", "Bloch phase k*a has to be real");
+    size_t n = mxGetNumberOfElements(bc);
+    if ((ndim == 2 && n != 1) || (ndim == 3 && n != 2))
+        fail("RGF:input", ndim == 2 ? "yBC must be a character vector or a numeric scalar"
+                                    : "xyBC must be a character vector or a numeric vector with 2 elements");
+    double* v = mxGetDoubles(bc);
+    p->bc[0] = p->bc[1] = RGF_BC_BLOCH;
+    p->bloch_ka[0] = v[0];
+    p->bloch_ka[1] = ndim == 3 ? v[1] : 0.0;
+}
+
+typedef struct {
+    double* re;
+    double* im;
+} eps_buf;
+
+static int read_syst(const mxArray* syst, rgf_problem* p, eps_buf* buf) {
+    if (!mxIsStruct(syst) || mxGetNumberOfElements(syst) != 1) fail("RGF:input", "syst must be a structure");
+    const char* req[] = {"epsilon", "epsilon_L", "epsilon_R", "k0dx"};
+    for (int q = 0; q < 4; ++q)
+        if (!mxGetField(syst, 0, req[q])) {
+            char b[96];
+            snprintf(b, sizeof b, "syst.%s must be provided", req[q]);
+            fail("RGF:input", b);
+        }
+    const mxArray* eps = mxGetField(syst, 0, "epsilon");
+    const mxArray* xy = mxGetField(syst, 0, "xyBC");
+    const mxArray* y = mxGetField(syst, 0, "yBC");
+    const mwSize nd = mxGetNumberOfDimensions(eps);
+    const mwSize* d = mxGetDimensions(eps);
+    memset(p, 0, sizeof *p);
+    int ndim = (xy && !y) || nd == 3 ? 3 : 2;
+    if (ndim == 3 && !xy) fail("RGF:input", "syst.xyBC must be provided");
+    if (ndim == 2 && !y) fail("RGF:input", "syst.yBC must be provided");
+    if (!mxIsDouble(eps)) fail("RGF:input", "syst.epsilon must be a double array");
+    p->ndim = ndim;
+    p->dims[0] = (int64_t)d[0];
+    p->dims[1] = (int64_t)d[1];
+    p->dims[2] = ndim == 3 ? (nd == 3 ? (int64_t)d[2] : 1) : 1;
+    size_t n = mxGetNumberOfElements(eps);
+    buf->re = NULL;
+    buf->im = NULL;
+    if (mxIsComplex(eps)) {
+        /* interleaved complex -> planar, as the C ABI expects */
+        mxComplexDouble* z = mxGetComplexDoubles(eps);
+        buf->re = (double*)mxMalloc(n * sizeof(double));
+        buf->im = (double*)mxMalloc(n * sizeof(double));
+        for (size_t k = 0; k < n; ++k) {
+            buf->re[k] = z[k].real;
+            buf->im[k] = z[k].imag;
+        }
+        p->eps_re = buf->re;
+        p->eps_im = buf->im;
+    } else {
+        p->eps_re = mxGetDoubles(eps);
+        p->eps_im = NULL;
+    }
+    p->eps_L[0] = scalar_part(mxGetField(syst, 0, "epsilon_L"), 0, "syst.epsilon_L");
+    p->eps_L[1] = scalar_part(mxGetField(syst, 0, "epsilon_L"), 1, "syst.epsilon_L");
+    p->eps_R[0] = scalar_part(mxGetField(syst, 0, "epsilon_R"), 0, "syst.epsilon_R");
+    p->eps_R[1] = scalar_part(mxGetField(syst, 0, "epsilon_R"), 1, "syst.epsilon_R");
+    p->k0dx[0] = scalar_part(mxGetField(syst, 0, "k0dx"), 0, "syst.k0dx");
+    p->k0dx[1] = scalar_part(mxGetField(syst, 0, "k0dx"), 1, "syst.k0dx");
+    parse_bc(ndim == 3 ? xy : y, ndim, p);
+    return ndim;
+}
+
+typedef struct {
+    int32_t* a[4];
+} sel_buf;
+
+/* cal_smatrix_RGF.m:103-175: cell {'left','L','right','R'} or struct of index vectors */
+static void read_side(const mxArray* spec, const char* kind, int32_t** ind_L, int32_t* n_L, int32_t** ind_R,
+                      int32_t* n_R, int32_t** keepL, int32_t** keepR) {
+    *ind_L = *ind_R = NULL;
+    *n_L = *n_R = 0;
+    if (mxIsCell(spec)) {
+        for (size_t k = 0; k < mxGetNumberOfElements(spec); ++k) {
+            const mxArray* el = mxGetCell(spec, k);
+            char s[16] = {0};
+            if (!el || !mxIsChar(el)) {
+                mexWarnMsgIdAndTxt("RGF:cell", "element %s{%d} has wrong format and is ignored; should be one of {'left', 'right', 'L', 'R'}", kind, (int)k + 1);
+                continue;
+            }
+            mxGetString(el, s, sizeof s);
+            if (!strcmp(s, "left") || !strcmp(s, "L")) *n_L = RGF_ALL;
+            else if (!strcmp(s, "right") || !strcmp(s, "R")) *n_R = RGF_ALL;
+            else mexWarnMsgIdAndTxt("RGF:cell", "element %s{%d} = %s ignored; should be one of {'left', 'right', 'L', 'R'}", kind, (int)k + 1, s);
+        }
+        return;
+    }
+    if (!mxIsStruct(spec)) {
+        char b[96];
+        snprintf(b, sizeof b, "input argument '%s' must be a structure or a cell array", kind);
+        fail("RGF:input", b);
+    }
+    const char* names[2] = {NULL, NULL};
+    char fL[16], fR[16];
+    snprintf(fL, sizeof fL, "ind_%s_L", kind);
+    snprintf(fR, sizeof fR, "ind_%s_R", kind);
+    names[0] = fL;
+    names[1] = fR;
+    for (int side = 0; side < 2; ++side) {
+        const mxArray* v = mxGetField(spec, 0, names[side]);
+        if (!v) continue;
+        size_t n = mxGetNumberOfElements(v);
+        int32_t* a = (int32_t*)mxMalloc((n ? n : 1) * sizeof(int32_t));
+        double* src = mxIsDouble(v) ? mxGetDoubles(v) : NULL;
+        for (size_t q = 0; q < n; ++q) {
+            double x = src ? src[q] : 0.0;
+            if (!src || x != (double)(int32_t)x) {
+                /* non-integers are rejected like out-of-range indices (cal_smatrix_RGF.m:110) */
+                a[q] = 0;
+            } else {
+                a[q] = (int32_t)x;
+            }
+        }
+        if (side == 0) {
+            *ind_L = a;
+            *n_L = (int32_t)n;
+            *keepL = a;
+        } else {
+            *ind_R = a;
+            *n_R = (int32_t)n;
+            *keepR = a;
+        }
+    }
+}
+
+static void get_opts(const mxArray* s, rgf_options* o, int32_t* every) {
+    rgf_default_options(o);
+    *every = 32;
+    if (!s || !mxIsStruct(s)) return;
+    const mxArray* f;
+    if ((f = mxGetField(s, 0, "precision")) && mxIsChar(f)) {
+        char b[32];
+        mxGetString(f, b, sizeof b);
+        o->precision = streq_ci(b, "fp64") ? RGF_PRECISION_FP64 : RGF_PRECISION_FP32_REFINED;
+    }
+    if ((f = mxGetField(s, 0, "refine_tol"))) o->refine_tol = mxGetScalar(f);
+    if ((f = mxGetField(s, 0, "max_refine_iter"))) o->max_refine_iter = (int32_t)mxGetScalar(f);
+    if ((f = mxGetField(s, 0, "checkpoint_every"))) *every = (int32_t)mxGetScalar(f);
+}
+
+void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+    int gradient = nrhs >= 1 && mxIsChar(prhs[0]);
+    int a0 = gradient ? 1 : 0;
+    if (nrhs < a0 + 3) fail("RGF:input", "usage: cal_smatrix_RGF_mex(syst, out, in [, opts])");
+    if (!g_handle) {
+        rgf_status st = rgf_create(&g_handle, -1);
+        if (st != RGF_OK) fail("RGF:device", rgf_last_error(NULL));
+        mexAtExit(release_handle);
+    }
+    rgf_problem p;
+    eps_buf eb;
+    read_syst(prhs[a0], &p, &eb);
+    rgf_selection sel;
+    int32_t *k1 = NULL, *k2 = NULL, *k3 = NULL, *k4 = NULL;
+    read_side(prhs[a0 + 1], "out", (int32_t**)&sel.ind_out_L, &sel.n_out_L, (int32_t**)&sel.ind_out_R,
+              &sel.n_out_R, &k1, &k2);
+    read_side(prhs[a0 + 2], "in", (int32_t**)&sel.ind_in_L, &sel.n_in_L, (int32_t**)&sel.ind_in_R, &sel.n_in_R,
+              &k3, &k4);
+    rgf_options opt;
+    int32_t every;
+    get_opts(nrhs > a0 + 3 ? prhs[a0 + 3] : NULL, &opt, &every);
+    rgf_info info;
+    if (!gradient) {
+        int32_t nL, nR, nout, nin;
+        check(rgf_query(&p, &sel, &nL, &nR, &nout, &nin), NULL);
+        if (nout == 0 || nin == 0) {
+            plhs[0] = mxCreateDoubleMatrix(nout, nin, mxREAL); /* cal_smatrix_RGF.m:181-184 */
+        } else {
+            plhs[0] = mxCreateDoubleMatrix(nout, nin, mxCOMPLEX);
+            check(rgf_solve(g_handle, &p, &sel, &opt, (double*)mxGetComplexDoubles(plhs[0]), &info), g_handle);
+        }
+    } else {
+        size_t n = (size_t)(p.dims[0] * p.dims[1] * p.dims[2]);
+        double T = 0;
+        mxArray* g = mxCreateNumericArray(p.ndim == 3 ? 3 : 2, (const mwSize[]){(mwSize)p.dims[0], (mwSize)p.dims[1], (mwSize)p.dims[2]}, mxDOUBLE_CLASS, p.eps_im ? mxCOMPLEX : mxREAL);
+        double* gre = (double*)mxMalloc(n * sizeof(double));
+        double* gim = (double*)mxMalloc(n * sizeof(double));
+        check(rgf_transmission_gradient(g_handle, &p, &sel, &opt, every, &T, gre, gim, &info), g_handle);
+        if (p.eps_im) {
+            mxComplexDouble* z = mxGetComplexDoubles(g);
+            for (size_t k = 0; k < n; ++k) {
+                z[k].real = gre[k];
+                z[k].imag = gim[k];
+            }
+        } else {
+            memcpy(mxGetDoubles(g), gre, n * sizeof(double));
+        }
+        mxFree(gre);
+        mxFree(gim);
+        plhs[0] = mxCreateDoubleScalar(T);
+        if (nlhs > 1) plhs[1] = g;
+        else mxDestroyArray(g);
+    }
+    if (!gradient && nlhs > 1) {
+        const char* f[] = {"n_prop_L", "n_prop_R", "refine_iters", "refine_residual", "seconds_total"};
+        plhs[1] = mxCreateStructMatrix(1, 1, 5, f);
+        mxSetField(plhs[1], 0, "n_prop_L", mxCreateDoubleScalar(info.n_prop_L));
+        mxSetField(plhs[1], 0, "n_prop_R", mxCreateDoubleScalar(info.n_prop_R));
+        mxSetField(plhs[1], 0, "refine_iters", mxCreateDoubleScalar(info.refine_iters));
+        mxSetField(plhs[1], 0, "refine_residual", mxCreateDoubleScalar(info.refine_residual));
+        mxSetField(plhs[1], 0, "seconds_total", mxCreateDoubleScalar(info.seconds_total));
+    }
+    if (k1) mxFree(k1);
+    if (k2) mxFree(k2);
+    if (k3) mxFree(k3);
+    if (k4) mxFree(k4);
+    if (eb.re) mxFree(eb.re);
+    if (eb.im) mxFree(eb.im);
+}
