@@ -495,11 +495,17 @@ __global__ __launch_bounds__(512) void gj_diag_blk_kernel(T* Are, T* Aim, int64_
     static_assert(NB % 64 == 0, "NB must be a multiple of 64");
     __shared__ T sr[NB][LDP], si[NB][LDP];
     __shared__ T orr[PW][NB], ori[PW][NB];  // M_old[P, :] for the block sweep
+    constexpr int TPRW = NT / NB;    // phase-1 threads per row
+    constexpr int CP = PW / TPRW;    // phase-1 panel columns per thread
+    static_assert(TPRW * CP == PW, "bad phase-1 tiling");
     __shared__ int pivs[NB];
-    __shared__ int bad_s;
+    __shared__ int bad_s, any_swap;
+    __shared__ T colr[2][NB], coli[2][NB], rowr[2][PW], rowi[2][PW], pvr[PW], pvi[PW];
+    __shared__ float gstat[3][NT / 64];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const int prow = tid / TPRW, pc0 = (tid % TPRW) * CP;
     if (tid == 0) bad_s = 0;
     for (int e = tid; e < NB * NB; e += NT) {
         const int i = e / NB, j = e % NB;
@@ -513,137 +519,148 @@ __global__ __launch_bounds__(512) void gj_diag_blk_kernel(T* Are, T* Aim, int64_
     }
     __syncthreads();
     for (int q0 = 0; q0 < NB; q0 += PW) {
-        // ---- phase 1: wavefront 0 factors the panel [q0, q0+16).  First
-        // without row interchanges (the slice matrices are close to diagonally
-        // dominant); if the panel grows by more than 1e3 the attempt is
-        // discarded and the panel is factored again with partial pivoting.
-        if (wave == 0) {
-            T xr[RPL][PW], xi[RPL][PW];
-            int bad = 0;
-            float m0 = 0.f;
+        // ---- phase 1: all threads factor the panel [q0, q0+16): thread t
+        // owns CP panel columns of row t / TPRW.  Each pivot step publishes
+        // the pivot row and the pivot column through LDS (double-buffered,
+        // one barrier per step).  The first attempt uses no row interchanges
+        // (the slice matrices are close to diagonally dominant); if the panel
+        // grows by more than 1e3 it is factored again with partial pivoting.
+        {
+            T xr[CP], xi[CP];
+            bool pivoting = false;
+            bool swapped = false;
 #pragma unroll 1
             for (int attempt = 0; attempt < 2; ++attempt) {
-                const bool pivoting = attempt == 1;
+                float m0 = 0.f;
 #pragma unroll
-                for (int s = 0; s < RPL; ++s)
-#pragma unroll
-                    for (int c = 0; c < PW; ++c) {
-                        xr[s][c] = sr[lane + 64 * s][q0 + c];
-                        xi[s][c] = si[lane + 64 * s][q0 + c];
-                        if (attempt == 0) m0 = fmaxf(m0, (float)(xr[s][c] * xr[s][c] + xi[s][c] * xi[s][c]));
-                    }
-                bad = 0;
+                for (int c = 0; c < CP; ++c) {
+                    xr[c] = sr[prow][q0 + pc0 + c];
+                    xi[c] = si[prow][q0 + pc0 + c];
+                    m0 = fmaxf(m0, (float)(xr[c] * xr[c] + xi[c] * xi[c]));
+                }
+                int bad = 0;
+                swapped = false;
 #pragma unroll
                 for (int k = 0; k < PW; ++k) {
-                    const int kc = q0 + k;
+                    const int kc = q0 + k, bsel = k & 1;
+                    // publish column k (its owner in every row) and row kc
+                    if ((tid % TPRW) == k / CP) {
+                        colr[bsel][prow] = xr[k % CP];
+                        coli[bsel][prow] = xi[k % CP];
+                    }
+                    if (prow == kc) {
+#pragma unroll
+                        for (int c = 0; c < CP; ++c) {
+                            rowr[bsel][pc0 + c] = xr[c];
+                            rowi[bsel][pc0 + c] = xi[c];
+                        }
+                    }
+                    __syncthreads();
                     int piv = kc;
                     if (pivoting) {
-                        // largest |x| among rows >= kc, lowest row on ties
+                        // every wavefront finds the same pivot: largest |x| among rows >= kc
                         unsigned long long key = 0;
-#pragma unroll
-                        for (int s = 0; s < RPL; ++s) {
-                            const int row = lane + 64 * s;
-                            const float m = (float)(xr[s][k] * xr[s][k] + xi[s][k] * xi[s][k]);
+                        for (int rr2 = kc + lane; rr2 < NB; rr2 += 64) {
+                            const T a = colr[bsel][rr2], b = coli[bsel][rr2];
+                            const float m = (float)(a * a + b * b);
                             const unsigned long long kk = ((unsigned long long)__float_as_uint(m) << 32) |
-                                                          (unsigned long long)(0xffffffffu - (unsigned)row);
-                            if (row >= kc && kk > key) key = kk;
+                                                          (unsigned long long)(0xffffffffu - (unsigned)rr2);
+                            if (kk > key) key = kk;
                         }
                         key = wave_max_u64(key);
                         piv = (int)(0xffffffffu - (unsigned)(key & 0xffffffffu));
-                    }
-                    const int la = kc & 63, sa = kc >> 6, lb = piv & 63, sb = piv >> 6;
-                    // pivot value (row piv, column k) and its inverse
-                    T vk_r = xr[0][k], vk_i = xi[0][k];
+                        if (piv != kc) {  // uniform
+                            swapped = true;
+                            if (prow == piv) {
 #pragma unroll
-                    for (int s = 1; s < RPL; ++s)
-                        if (sb == s) {
-                            vk_r = xr[s][k];
-                            vk_i = xi[s][k];
+                                for (int c = 0; c < CP; ++c) {
+                                    pvr[pc0 + c] = xr[c];
+                                    pvi[pc0 + c] = xi[c];
+                                }
+                            }
+                            __syncthreads();
                         }
-                    const T pkr = lane_bcast(vk_r, lb), pki = lane_bcast(vk_i, lb);
+                    }
+                    const bool xch = pivoting && piv != kc;
+                    const T pkr = xch ? pvr[k] : rowr[bsel][k];
+                    const T pki = xch ? pvi[k] : rowi[bsel][k];
                     const T den = pkr * pkr + pki * pki;
                     if (!(den > 0) || !isfinite(den)) bad = 1;
                     const T ivr = pkr / den, ivi = -pki / den;
-                    // scaled pivot row; with pivoting, row piv receives the old row kc
-                    T rr[PW], ri[PW];
+                    if (prow == kc) {
 #pragma unroll
-                    for (int c = 0; c < PW; ++c) {
-                        T vb_r = xr[0][c], vb_i = xi[0][c], va_r = xr[0][c], va_i = xi[0][c];
-#pragma unroll
-                        for (int s = 1; s < RPL; ++s) {
-                            if (sb == s) {
-                                vb_r = xr[s][c];
-                                vb_i = xi[s][c];
-                            }
-                            if (sa == s) {
-                                va_r = xr[s][c];
-                                va_i = xi[s][c];
-                            }
+                        for (int c = 0; c < CP; ++c) {
+                            const int j = pc0 + c;
+                            const T br = xch ? pvr[j] : rowr[bsel][j], bi2 = xch ? pvi[j] : rowi[bsel][j];
+                            xr[c] = (j == k) ? ivr : br * ivr - bi2 * ivi;
+                            xi[c] = (j == k) ? ivi : br * ivi + bi2 * ivr;
                         }
-                        const T br = lane_bcast(vb_r, lb), bi2 = lane_bcast(vb_i, lb);
-                        rr[c] = (c == k) ? ivr : br * ivr - bi2 * ivi;
-                        ri[c] = (c == k) ? ivi : br * ivi + bi2 * ivr;
-                        if (pivoting && piv != kc) {
-                            const T ar = lane_bcast(va_r, la), ai = lane_bcast(va_i, la);
+                    } else {
+                        // row piv receives the old row kc before its elimination
+                        const bool moved = xch && prow == piv;
+                        const T fr = moved ? colr[bsel][kc] : colr[bsel][prow];
+                        const T fi = moved ? coli[bsel][kc] : coli[bsel][prow];
+                        const T gr = fr * ivr - fi * ivi, gi = fr * ivi + fi * ivr;  // f / pivot
 #pragma unroll
-                            for (int s = 0; s < RPL; ++s)
-                                if (lane + 64 * s == piv) {
-                                    xr[s][c] = ar;
-                                    xi[s][c] = ai;
-                                }
-                        }
-                    }
-#pragma unroll
-                    for (int s = 0; s < RPL; ++s) {
-                        const int row = lane + 64 * s;
-                        if (row == kc) {
-#pragma unroll
-                            for (int c = 0; c < PW; ++c) {
-                                xr[s][c] = rr[c];
-                                xi[s][c] = ri[c];
-                            }
-                        } else {
-                            const T fr = xr[s][k], fi = xi[s][k];
-#pragma unroll
-                            for (int c = 0; c < PW; ++c) {
-                                const T br = (c == k) ? T(0) : xr[s][c];
-                                const T bi2 = (c == k) ? T(0) : xi[s][c];
-                                xr[s][c] = br - (fr * rr[c] - fi * ri[c]);
-                                xi[s][c] = bi2 - (fr * ri[c] + fi * rr[c]);
+                        for (int c = 0; c < CP; ++c) {
+                            const int j = pc0 + c;
+                            T br = moved ? rowr[bsel][j] : xr[c];
+                            T bi2 = moved ? rowi[bsel][j] : xi[c];
+                            const T yr = xch ? pvr[j] : rowr[bsel][j], yi = xch ? pvi[j] : rowi[bsel][j];
+                            if (j == k) {
+                                xr[c] = -gr;
+                                xi[c] = -gi;
+                            } else {
+                                xr[c] = br - (gr * yr - gi * yi);
+                                xi[c] = bi2 - (gr * yi + gi * yr);
                             }
                         }
                     }
-                    if (lane == 0) pivs[kc] = piv;
+                    if (tid == 0) pivs[kc] = piv;
                 }
-                if (pivoting) break;
-                // growth check of the unpivoted attempt (one reduction per panel)
+                if (pivoting) {
+                    if (bad) bad_s = 1;
+                    break;
+                }
+                // growth check of the unpivoted attempt
                 float m1 = 0.f;
 #pragma unroll
-                for (int s = 0; s < RPL; ++s)
+                for (int c = 0; c < CP; ++c) m1 = fmaxf(m1, (float)(xr[c] * xr[c] + xi[c] * xi[c]));
+                const unsigned w1 = (unsigned)wave_max_u64(__float_as_uint(m1));
+                const unsigned w0 = (unsigned)wave_max_u64(__float_as_uint(m0));
+                const unsigned wb = (unsigned)wave_max_u64((unsigned)bad);
+                if (lane == 0) {
+                    gstat[0][wave] = __uint_as_float(w0);
+                    gstat[1][wave] = __uint_as_float(w1);
+                    gstat[2][wave] = (float)wb;
+                }
+                __syncthreads();
+                float g0 = 0.f, g1 = 0.f, gb = 0.f;
 #pragma unroll
-                    for (int c = 0; c < PW; ++c)
-                        m1 = fmaxf(m1, (float)(xr[s][c] * xr[s][c] + xi[s][c] * xi[s][c]));
-                const unsigned long long g =
-                    wave_max_u64(((unsigned long long)__float_as_uint(m1) << 32) | __float_as_uint(m0));
-                const float gm1 = __uint_as_float((unsigned)(g >> 32));
-                float gm0 = __uint_as_float((unsigned)(wave_max_u64(__float_as_uint(m0)) & 0xffffffffu));
-                const bool ok = !bad && isfinite(gm1) && gm1 <= 1e6f * fmaxf(gm0, 1e-30f);
-                if (ok) break;
+                for (int w = 0; w < NT / 64; ++w) {
+                    g0 = fmaxf(g0, gstat[0][w]);
+                    g1 = fmaxf(g1, gstat[1][w]);
+                    gb = fmaxf(gb, gstat[2][w]);
+                }
+                __syncthreads();
+                if (gb == 0.f && isfinite(g1) && g1 <= 1e6f * fmaxf(g0, 1e-30f)) break;
+                pivoting = true;
             }
 #pragma unroll
-            for (int s = 0; s < RPL; ++s)
-#pragma unroll
-                for (int c = 0; c < PW; ++c) {
-                    sr[lane + 64 * s][q0 + c] = xr[s][c];
-                    si[lane + 64 * s][q0 + c] = xi[s][c];
-                }
-            if (bad && lane == 0) bad_s = 1;
+            for (int c = 0; c < CP; ++c) {
+                sr[prow][q0 + pc0 + c] = xr[c];
+                si[prow][q0 + pc0 + c] = xi[c];
+            }
+            if (!pivoting) {
+                // nothing to interchange in phase 2a
+            }
+            if (tid == 0) any_swap = swapped ? 1 : 0;
         }
         __syncthreads();
         // ---- phase 2a: row interchanges on the other columns; keep M_old[P, :]
         for (int j = tid; j < NB; j += NT) {
             if (j >= q0 && j < q0 + PW) continue;
-            for (int k = 0; k < PW; ++k) {
+            for (int k = 0; any_swap && k < PW; ++k) {
                 const int a = q0 + k, b = pivs[a];
                 if (b != a) {
                     const T tr = sr[a][j], ti = si[a][j];
