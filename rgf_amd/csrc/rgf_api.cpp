@@ -45,6 +45,7 @@ struct rgf_context {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    Overlap ov;  // look-ahead stream and events of the slice inverse
     std::string err;
     std::map<std::string, DevBuf> bufs;
     int* d_status = nullptr;
@@ -330,7 +331,7 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
                                    cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<double>(s, N, G, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
-        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer);
+        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer, &c->ov);
         if (pi) {
             cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             std::swap(X, X2);
@@ -395,7 +396,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
                                   cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<float>(s, N, Fn(n), nullptr, nullptr, w.epsR, w.k2, &G0R32, w.A0);
-        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer);
+        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer, &c->ov);
     }
     RGF_HIP(hipStreamSynchronize(s));
     w.t_slices = now() - t0;
@@ -527,7 +528,7 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
                                    cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<double>(s, N, out, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
-        invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer);
+        invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer, &c->ov);
     };
     reset_status(c);
     RGF_HIP(hipStreamSynchronize(s));
@@ -836,6 +837,8 @@ rgf_status rgf_create(rgf_handle* h, int32_t device) {
         RGF_HIP(hipSetDevice(device));
         RGF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
+        RGF_HIP(hipStreamCreateWithFlags(&c->ov.s2, hipStreamNonBlocking));
+        for (auto& e : c->ov.ev) RGF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         RGF_HIP(hipMalloc(&c->d_status, 256));
         RGF_HIP(hipMalloc(&c->d_scratch, 256));
         *h = c.release();
@@ -853,6 +856,9 @@ rgf_status rgf_destroy(rgf_handle h) {
         if (h->d_status) RGF_HIP(hipFree(h->d_status));
         if (h->d_scratch) RGF_HIP(hipFree(h->d_scratch));
         if (h->own) RGF_HIP(hipStreamDestroy(h->own));
+        for (auto& e : h->ov.ev)
+            if (e) RGF_HIP(hipEventDestroy(e));
+        if (h->ov.s2) RGF_HIP(hipStreamDestroy(h->ov.s2));
     });
     delete h;
     return st;
@@ -982,13 +988,13 @@ rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double*
         if (precision == RGF_PRECISION_FP64) {
             const int b = block_size > 0 ? block_size : default_block(precision);
             double* ws = dbuf<double>(h, "inv_ws", inverse_workspace_elems<double>(n, b));
-            invert_inplace<double>(h->stream, n, M, b, ws, h->d_status);
+            invert_inplace<double>(h->stream, n, M, b, ws, h->d_status, 0, nullptr, &h->ov);
         } else {
             const int b = block_size > 0 ? block_size : default_block(precision);
             CMat<float> M32 = dmat<float>(h, "dense_A32", n, n);
             convert<float, double>(h->stream, n, n, M, M32, 1.0);
             float* ws = dbuf<float>(h, "inv_ws32", inverse_workspace_elems<float>(n, b));
-            invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status);
+            invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status, 0, nullptr, &h->ov);
             convert<double, float>(h->stream, n, n, M32, M, 1.0);
         }
         check_status(h);

@@ -62,6 +62,12 @@ struct KernelTimer {
     ~KernelTimer();
 };
 
+// Second stream and events for the look-ahead of the slice inverse.
+struct Overlap {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
 // ---- kernel launchers (rgf_kernels.hip) -------------------------------------
 // C = alpha * A(MxK) * B(KxN) + beta * C, beta in {0,1}.  Tiles overlapping
 // rows or columns [skip_lo, skip_hi) are left untouched (block Gauss-Jordan).
@@ -80,7 +86,7 @@ template <typename T>
 size_t inverse_workspace_elems(int64_t N, int b);
 template <typename T>
 void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_status, int use3m = 0,
-                    KernelTimer* timer = nullptr);
+                    KernelTimer* timer = nullptr, const Overlap* ov = nullptr);
 
 // M <- A0 - diag(k2*eps) - M - Gextra  (Gextra may be null).  eps is a
 // planar FP64 vector (re, im; im may be null) or, when eps_re is null, the

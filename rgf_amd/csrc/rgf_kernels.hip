@@ -26,6 +26,11 @@ static int grid_for(int64_t total, int threads = 256, int cap = 16384) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((total + threads - 1) / threads, cap));
 }
 
+// Rows [rlo, rhi) and columns [clo, chi) of C that a GEMM leaves untouched.
+struct Skip {
+    int64_t rlo = 0, rhi = 0, clo = 0, chi = 0;
+};
+
 // ------------------------------------------------------------------ MFMA wrappers
 template <typename T>
 struct Mfma;
@@ -72,11 +77,47 @@ struct Vec<double> {
     static constexpr int W = 2;
 };
 
+// One GEMM problem (planar operands, row-major).
+template <typename T>
+struct GemmArgs {
+    const T* Are;
+    const T* Aim;
+    int64_t lda;
+    const T* Bre;
+    const T* Bim;
+    int64_t ldb;
+    T* Cre;
+    T* Cim;
+    int64_t ldc;
+    int M, N, K;
+    double alpha_re, alpha_im;
+    int beta;
+    int srlo, srhi, sclo, schi;
+    int vec_ok;
+    int tiles_n, tiles;  // tile grid of this problem (set at launch)
+};
+
+// A 1D grid covers the tiles of g0 followed by those of g1: two independent
+// problems sharing the tile configuration run as one launch.
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool USE3M>
-__global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
-    const T* __restrict__ Are, const T* __restrict__ Aim, int64_t lda, const T* __restrict__ Bre,
-    const T* __restrict__ Bim, int64_t ldb, T* Cre, T* Cim, int64_t ldc, int M, int N, int K,
-    double alpha_re, double alpha_im, int beta, int skip_lo, int skip_hi, int vec_ok) {
+__global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, GemmArgs<T> g1) {
+    const bool second = (int)blockIdx.x >= g0.tiles;
+    const GemmArgs<T>& g = second ? g1 : g0;
+    const int lin = second ? (int)blockIdx.x - g0.tiles : (int)blockIdx.x;
+    const T* __restrict__ Are = g.Are;
+    const T* __restrict__ Aim = g.Aim;
+    const int64_t lda = g.lda;
+    const T* __restrict__ Bre = g.Bre;
+    const T* __restrict__ Bim = g.Bim;
+    const int64_t ldb = g.ldb;
+    T* Cre = g.Cre;
+    T* Cim = g.Cim;
+    const int64_t ldc = g.ldc;
+    const int M = g.M, N = g.N, K = g.K;
+    const double alpha_re = g.alpha_re, alpha_im = g.alpha_im;
+    const int beta = g.beta;
+    const int srlo = g.srlo, srhi = g.srhi, sclo = g.sclo, schi = g.schi;
+    const int vec_ok = g.vec_ok;
     using V = typename Mfma<T>::v4;
     using VT = typename Vec<T>::type;
     constexpr int VW = Vec<T>::W;
@@ -87,13 +128,14 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
     __shared__ T sA[NP][BM][BK + 1];
     __shared__ T sB[NP][BK][BN + 1];
 
-    const int tile_n = blockIdx.x * BN;
-    const int tile_m = blockIdx.y * BM;
+    const int tile_n = (lin % g.tiles_n) * BN;
+    const int tile_m = (lin / g.tiles_n) * BM;
+    if (tile_m >= M || tile_n >= N) return;
     // tiles lying entirely inside the skipped band exit; partially covered
     // tiles are masked element-wise in the epilogue
-    if (skip_hi > skip_lo) {
-        const bool rs = tile_m >= skip_lo && tile_m + BM <= skip_hi;
-        const bool cs = tile_n >= skip_lo && tile_n + BN <= skip_hi;
+    {
+        const bool rs = tile_m >= srlo && tile_m + BM <= srhi;
+        const bool cs = tile_n >= sclo && tile_n + BN <= schi;
         if (rs || cs) return;
     }
     const int tid = threadIdx.x;
@@ -314,7 +356,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(
             for (int r = 0; r < 4; ++r) {
                 const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(l4, r);
                 const int gj = tile_n + wn * TN + b * 16 + l16;
-                const bool masked = (gi >= skip_lo && gi < skip_hi) || (gj >= skip_lo && gj < skip_hi);
+                const bool masked = (gi >= srlo && gi < srhi) || (gj >= sclo && gj < schi);
                 if (gi < M && gj < N && !masked) {
                     T pr, pi;
                     if (USE3M) {
@@ -355,23 +397,61 @@ struct GemmCfg<float> {
     static constexpr int BK_SMALL = 32, BK_BIG = 32;
 };
 
-template <typename T, int BM, int BN, int BK, int WM, int WN>
-static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C,
-                         cd alpha, int beta, int64_t skip_lo, int64_t skip_hi, int use3m) {
-    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM));
-    dim3 block(WM * WN * 64);
+template <typename T>
+static GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha, int beta,
+                             Skip sk) {
     // vector loads need 16-byte aligned rows in A and B
     const int VW = 16 / (int)sizeof(T);
-    const int vec_ok = (A.ld % VW == 0) && (B.ld % VW == 0) && ((uintptr_t)A.re % 16 == 0) &&
-                       ((uintptr_t)A.im % 16 == 0) && ((uintptr_t)B.re % 16 == 0) && ((uintptr_t)B.im % 16 == 0);
+    GemmArgs<T> g;
+    g.Are = A.re;
+    g.Aim = A.im;
+    g.lda = A.ld;
+    g.Bre = B.re;
+    g.Bim = B.im;
+    g.ldb = B.ld;
+    g.Cre = C.re;
+    g.Cim = C.im;
+    g.ldc = C.ld;
+    g.M = (int)M;
+    g.N = (int)N;
+    g.K = (int)K;
+    g.alpha_re = alpha.real();
+    g.alpha_im = alpha.imag();
+    g.beta = beta;
+    g.srlo = (int)sk.rlo;
+    g.srhi = (int)sk.rhi;
+    g.sclo = (int)sk.clo;
+    g.schi = (int)sk.chi;
+    g.vec_ok = (A.ld % VW == 0) && (B.ld % VW == 0) && ((uintptr_t)A.re % 16 == 0) && ((uintptr_t)A.im % 16 == 0) &&
+               ((uintptr_t)B.re % 16 == 0) && ((uintptr_t)B.im % 16 == 0);
+    return g;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_, int use3m) {
+    GemmArgs<T> g0 = g0_, g1 = g1_ ? *g1_ : g0_;
+    auto tiles = [](GemmArgs<T>& g) {
+        g.tiles_n = (g.N + BN - 1) / BN;
+        g.tiles = g.tiles_n * ((g.M + BM - 1) / BM);
+        if (g.M <= 0 || g.N <= 0) g.tiles = 0;
+    };
+    tiles(g0);
+    tiles(g1);
+    const int64_t nblocks = (int64_t)g0.tiles + (g1_ ? g1.tiles : 0);
+    if (nblocks <= 0) return;
+    dim3 grid((unsigned)nblocks);
+    dim3 block(WM * WN * 64);
     if (use3m)
-        hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, true>), grid, block, 0, s, A.re, A.im, A.ld,
-                           B.re, B.im, B.ld, C.re, C.im, C.ld, (int)M, (int)N, (int)K, alpha.real(),
-                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi, vec_ok);
+        hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, true>), grid, block, 0, s, g0, g1);
     else
-        hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, false>), grid, block, 0, s, A.re, A.im, A.ld,
-                           B.re, B.im, B.ld, C.re, C.im, C.ld, (int)M, (int)N, (int)K, alpha.real(),
-                           alpha.imag(), beta, (int)skip_lo, (int)skip_hi, vec_ok);
+        hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, false>), grid, block, 0, s, g0, g1);
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C,
+                         cd alpha, int beta, Skip sk, int use3m) {
+    const GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, C, alpha, beta, sk);
+    launch_pair<T, BM, BN, BK, WM, WN>(s, g, nullptr, use3m);
 }
 
 // C = alpha * sum_z P_z + beta * C over the split-K partial products
@@ -399,11 +479,12 @@ __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int nz, const T* P, i
 }
 
 template <typename T>
-void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
-           int beta, int64_t skip_lo, int64_t skip_hi, int use3m, T* ws, size_t ws_elems) {
+void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
+                int beta, Skip sk, int use3m, T* ws, size_t ws_elems) {
     if (M <= 0 || N <= 0) return;
+    const bool any_skip = sk.rhi > sk.rlo || sk.chi > sk.clo;
     // split-K for thin products with a long inner dimension
-    if (ws && skip_hi <= skip_lo && K >= 1024) {
+    if (ws && !any_skip && K >= 1024) {
         const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
         if (tiles < 512) {
             const int64_t per = 2 * M * N;  // elements of one partial (two planes)
@@ -418,7 +499,7 @@ void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B,
                     CMat<T> Az(A.re + k0, A.im + k0, A.ld);
                     CMat<T> Bz(B.re + k0 * B.ld, B.im + k0 * B.ld, B.ld);
                     CMat<T> Pz(ws + z * per, ws + z * per + M * N, N);
-                    launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, kl, Az, Bz, Pz, cd(1, 0), 0, 0, 0,
+                    launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, kl, Az, Bz, Pz, cd(1, 0), 0, Skip{},
                                                                         use3m);
                 }
                 hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
@@ -435,13 +516,28 @@ void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B,
     if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX) throw Error(RGF_ERR_INVALID_ARGUMENT, "GEMM too large");
     const int64_t tiles_big = ((M + 127) / 128) * ((N + 127) / 128);
     const int64_t tiles_mid = ((M + 63) / 64) * ((N + 63) / 64);
-    if (tiles_big >= 1024)
-        launch_cgemm<T, 128, 128, GemmCfg<T>::BK_BIG, 2, 4>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
+    // FP64 runs best on 128x128 tiles (8 waves); FP32 on 64x64 tiles, which
+    // keep two or three work groups per CU
+    if (sizeof(T) == 8 && tiles_big >= 1024)
+        launch_cgemm<T, 128, 128, GemmCfg<T>::BK_BIG, 2, 4>(s, M, N, K, A, B, C, alpha, beta, sk, use3m);
     else if (tiles_mid >= 512)
-        launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
+        launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, K, A, B, C, alpha, beta, sk, use3m);
     else
-        launch_cgemm<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, M, N, K, A, B, C, alpha, beta, skip_lo, skip_hi, use3m);
+        launch_cgemm<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, M, N, K, A, B, C, alpha, beta, sk, use3m);
     RGF_HIP(hipGetLastError());
+}
+
+// Two small products in one launch on 32x32 tiles.
+template <typename T>
+static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, int use3m) {
+    launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
+    RGF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
+           int beta, int64_t skip_lo, int64_t skip_hi, int use3m, T* ws, size_t ws_elems) {
+    cgemm_skip<T>(s, M, N, K, A, B, C, alpha, beta, Skip{skip_lo, skip_hi, skip_lo, skip_hi}, use3m, ws, ws_elems);
 }
 
 // ------------------------------------------------------------------ K2: diagonal-block inverse
@@ -450,13 +546,13 @@ void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B,
 // row interchanges confined to the block, writes D^-1 back to A and to Dout
 // (ld = ldd), and folds the column interchanges that undo the row pivoting
 // into the write-back.
-// The nb x nb block (padded with the identity to
-// NB) lives in LDS.  For each 16-column panel, wavefront 0 keeps the panel
-// (all NB rows) in registers and runs 16 Gauss-Jordan steps with partial
-// pivoting over all remaining rows, broadcasting the pivot row with
-// readlane (no barriers inside a panel).  The other columns are then
-// brought up to date by all wavefronts: the 16 row interchanges, then the
-// rank-16 block sweep  M[:, J] = (rows P ? 0 : M[:, J]) + M'[:, P] * M_old[P, J]
+// The nb x nb block (padded with the identity to NB) lives in LDS.  For
+// each 16-column panel, all wavefronts run 16 Gauss-Jordan steps on the
+// panel (each thread holds a few panel columns of one row in registers; the
+// pivot row and column go through LDS, one barrier per step).  The other
+// columns are then brought up to date: the 16 row interchanges, if any,
+// then the rank-16 block sweep
+//   M[:, J] = (rows P ? 0 : M[:, J]) + M'[:, P] * M_old[P, J]
 // on MFMA, where M'[:, P] are the panel's new values.
 template <typename T>
 __device__ __forceinline__ T lane_bcast(T v, int l);
@@ -735,26 +831,28 @@ __global__ __launch_bounds__(512) void gj_diag_blk_kernel(T* Are, T* Aim, int64_
     if (tid == 0 && bad_s) atomicOr(status, 1);
 }
 
-// Copy the new row panel R (nb x N, rows of Rbuf) into A[p0:p0+nb, :] and the
-// new column panel C (N x nb) into A[:, p0:p0+nb], skipping the diagonal block.
+// Partial write-back of block [p0, p0+nb): with rows != 0, the row panel
+// A[P, j] = R[:, j] (j outside P); and the column panel A[i, P] = C[i, :] for
+// rows i in [clo, chi) outside [slo, shi).
 template <typename T>
-__global__ void gj_writeback_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int p0, int nb, const T* Rre,
-                                    const T* Rim, int64_t ldr, const T* Cre, const T* Cim, int64_t ldc) {
-    const int64_t total = (int64_t)nb * N;
+__global__ void gj_writeback2_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int p0, int nb, const T* Rre,
+                                     const T* Rim, int64_t ldr, const T* Cre, const T* Cim, int64_t ldc, int rows,
+                                     int64_t clo, int64_t chi, int64_t slo, int64_t shi) {
+    const int64_t nrow = rows ? (int64_t)nb * N : 0;
+    const int64_t ncol = (chi - clo) * nb;
+    const int64_t total = nrow > ncol ? nrow : ncol;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
-        // row panel element (i, j)
-        {
+        if (e < nrow) {
             const int64_t i = e / N, j = e % N;
             if (j < p0 || j >= p0 + nb) {
                 Are[(p0 + i) * lda + j] = Rre[i * ldr + j];
                 Aim[(p0 + i) * lda + j] = Rim[i * ldr + j];
             }
         }
-        // column panel element (i, j)
-        {
-            const int64_t i = e / nb, j = e % nb;
-            if (i < p0 || i >= p0 + nb) {
+        if (e < ncol) {
+            const int64_t i = clo + e / nb, j = e % nb;
+            if (i < slo || i >= shi) {
                 Are[i * lda + p0 + j] = Cre[i * ldc + j];
                 Aim[i * lda + p0 + j] = Cim[i * ldc + j];
             }
@@ -765,12 +863,6 @@ __global__ void gj_writeback_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int 
 template <typename T>
 static constexpr int diag_bmax() {
     return sizeof(T) == 8 ? 64 : 128;  // the diagonal block lives in LDS
-}
-
-template <typename T>
-size_t inverse_workspace_elems(int64_t N, int b) {
-    // Rbuf (b x N) + Cbuf (N x b) + Dbuf (b x b), two planes each
-    return (size_t)2 * ((size_t)b * N * 2 + (size_t)b * b);
 }
 
 void KernelTimer::begin(hipStream_t s) {
@@ -807,46 +899,119 @@ KernelTimer::~KernelTimer() {
 }
 
 template <typename T>
+size_t inverse_workspace_elems(int64_t N, int b) {
+    // two sets of R (b x N) and C (N x b), plus D (b x b); two planes each
+    return (size_t)2 * (2 * (size_t)2 * b * N + (size_t)b * b);
+}
+
+template <typename T>
+static void launch_diag(hipStream_t s, CMat<T> A, int64_t p0, int nb, CMat<T> D, int* st) {
+    if constexpr (diag_bmax<T>() >= 128) {
+        if (nb > 64) {
+            hipLaunchKernelGGL((gj_diag_blk_kernel<T, 128>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb,
+                               D.re, D.im, D.ld, st);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb, D.re,
+                       D.im, D.ld, st);
+}
+
+// Blocked Gauss-Jordan (sweep operator) in place.  For pivot block P:
+//   D = A_PP^-1,  R = D A_P:,  C = -A_:P D,  A_JJ -= A_JP R_:J,  A_PJ = R_PJ,
+//   A_JP = C_JP,  A_PP = D.
+// With a second stream (look-ahead), the bulk update of step p runs there
+// while the first stream updates block row/column p+1, writes step p's panels
+// back and factors block p+1, so the single-work-group diagonal inverse and
+// the panel products overlap the bulk update.
+template <typename T>
 void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_status, int use3m,
-                    KernelTimer* timer) {
+                    KernelTimer* timer, const Overlap* ov) {
     if (N <= 0) return;
     if (b <= 0) b = diag_bmax<T>();
     if (b > diag_bmax<T>() || (b % 16) != 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "invalid inverse block size");
-    T* Rre = ws;
-    T* Rim = Rre + (size_t)b * N;
-    T* Cre = Rim + (size_t)b * N;
-    T* Cim = Cre + (size_t)N * b;
-    T* Dre = Cim + (size_t)N * b;
-    T* Dim = Dre + (size_t)b * b;
-    CMat<T> R(Rre, Rim, N), C(Cre, Cim, b), D(Dre, Dim, b);
-    for (int64_t p0 = 0; p0 < N; p0 += b) {
-        const int nb = (int)std::min<int64_t>(b, N - p0);
-        if constexpr (diag_bmax<T>() >= 128) {
-            if (nb > 64) {
-                hipLaunchKernelGGL((gj_diag_blk_kernel<T, 128>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0,
-                                   nb, D.re, D.im, D.ld, dev_status);
-            } else {
-                hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0,
-                                   nb, D.re, D.im, D.ld, dev_status);
-            }
-        } else {
-            hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb,
-                               D.re, D.im, D.ld, dev_status);
-        }
-        if (nb == N) break;  // single block: done
-        // R = D^-1 * A[P, :]
-        cgemm<T>(s, nb, N, nb, D, A.offset(p0, 0), R, cd(1, 0), 0, 0, 0, use3m);
-        // C = -A[:, P] * D^-1 (reads the old column panel)
-        cgemm<T>(s, N, nb, nb, A.offset(0, p0), D, C, cd(-1, 0), 0, 0, 0, use3m);
-        // A[J, J] -= A[J, P] * R[:, J]
-        if (timer) timer->begin(s);
-        cgemm<T>(s, N, N, nb, A.offset(0, p0), R, A, cd(-1, 0), 1, p0, p0 + nb, use3m);
-        if (timer) timer->end(s, (use3m ? 6.0 : 8.0) * (double)(N - nb) * (double)(N - nb) * nb);
-        const int64_t tot = (int64_t)nb * N;
-        const int nblk = (int)std::min<int64_t>((tot + 255) / 256, 4096);
-        hipLaunchKernelGGL((gj_writeback_kernel<T>), dim3(nblk), dim3(256), 0, s, A.re, A.im, A.ld, N, (int)p0, nb,
-                           R.re, R.im, R.ld, C.re, C.im, C.ld);
+    const size_t set = (size_t)2 * 2 * b * N;
+    CMat<T> R[2], C[2];
+    for (int q = 0; q < 2; ++q) {
+        T* base = ws + q * set;
+        R[q] = CMat<T>(base, base + (size_t)b * N, N);
+        C[q] = CMat<T>(base + 2 * (size_t)b * N, base + 3 * (size_t)b * N, b);
     }
+    CMat<T> D(ws + 2 * set, ws + 2 * set + (size_t)b * b, b);
+    const int64_t nblk = (N + b - 1) / b;
+    // diagonal block, then R = D A_P: and C = -A_:P D in one launch
+    auto panels = [&](hipStream_t st, int64_t p0, int nb, int q) {
+        launch_diag<T>(st, A, p0, nb, D, dev_status);
+        const GemmArgs<T> gr = gemm_args<T>(nb, N, nb, D, A.offset(p0, 0), R[q], cd(1, 0), 0, Skip{});
+        const GemmArgs<T> gc = gemm_args<T>(N, nb, nb, A.offset(0, p0), D, C[q], cd(-1, 0), 0, Skip{});
+        cgemm_pair<T>(st, gr, gc, use3m);
+    };
+    auto writeback = [&](hipStream_t st, int64_t p0, int nb, int q, int rows, int64_t clo, int64_t chi, int64_t slo,
+                         int64_t shi) {
+        const int64_t tot = std::max<int64_t>(rows ? (int64_t)nb * N : 0, (chi - clo) * nb);
+        if (tot <= 0) return;
+        hipLaunchKernelGGL((gj_writeback2_kernel<T>), dim3(grid_for(tot, 256, 4096)), dim3(256), 0, st, A.re, A.im,
+                           A.ld, N, (int)p0, nb, R[q].re, R[q].im, R[q].ld, C[q].re, C[q].im, C[q].ld, rows, clo, chi,
+                           slo, shi);
+    };
+    const double fl = use3m ? 6.0 : 8.0;
+    if (nblk == 1) {
+        launch_diag<T>(s, A, 0, (int)N, D, dev_status);
+        RGF_HIP(hipGetLastError());
+        return;
+    }
+    if (!ov || !ov->s2) {
+        for (int64_t p0 = 0; p0 < N; p0 += b) {
+            const int nb = (int)std::min<int64_t>(b, N - p0);
+            panels(s, p0, nb, 0);
+            if (timer) timer->begin(s);
+            cgemm<T>(s, N, N, nb, A.offset(0, p0), R[0], A, cd(-1, 0), 1, p0, p0 + nb, use3m);
+            if (timer) timer->end(s, fl * (double)(N - nb) * (double)(N - nb) * nb);
+            writeback(s, p0, nb, 0, 1, 0, N, p0, p0 + nb);
+        }
+        RGF_HIP(hipGetLastError());
+        return;
+    }
+    // Look-ahead.  Per block q (P = block q, Q = block q+1):
+    //  first stream : after the bulk update of q-1, update block row and
+    //                 column Q with step q (one launch, runs alone);
+    //  second stream: bulk update of everything outside P and Q, reading the
+    //                 old column panel in place, then write the column panel
+    //                 back outside P and Q;
+    //  first stream : meanwhile write the row panel and the Q rows of the
+    //                 column panel back, and factor block Q.
+    hipStream_t s2 = ov->s2;
+    panels(s, 0, (int)std::min<int64_t>(b, N), 0);
+    for (int64_t q = 0; q < nblk; ++q) {
+        const int64_t p0 = q * b;
+        const int nb = (int)std::min<int64_t>(b, N - p0);
+        const int st = (int)(q & 1);
+        const bool next = q + 1 < nblk;
+        const int64_t p1 = p0 + nb;
+        const int nb1 = next ? (int)std::min<int64_t>(b, N - p1) : 0;
+        if (q > 0) RGF_HIP(hipStreamWaitEvent(s, ov->ev[2 + (st ^ 1)], 0));
+        if (next) {
+            const GemmArgs<T> ga = gemm_args<T>(nb1, N, nb, A.offset(p1, p0), R[st], A.offset(p1, 0), cd(-1, 0), 1,
+                                                Skip{0, 0, p0, p1});
+            const GemmArgs<T> gb = gemm_args<T>(N, nb1, nb, A.offset(0, p0), R[st].offset(0, p1), A.offset(0, p1),
+                                                cd(-1, 0), 1, Skip{p0, p1 + nb1, 0, 0});
+            cgemm_pair<T>(s, ga, gb, use3m);
+        }
+        RGF_HIP(hipEventRecord(ov->ev[st], s));
+        RGF_HIP(hipStreamWaitEvent(s2, ov->ev[st], 0));
+        if (timer) timer->begin(s2);
+        cgemm_skip<T>(s2, N, N, nb, A.offset(0, p0), R[st], A, cd(-1, 0), 1, Skip{p0, p1 + nb1, p0, p1 + nb1}, use3m,
+                      nullptr, 0);
+        if (timer) {
+            const double nn = (double)(N - nb - nb1);
+            timer->end(s2, fl * nn * nn * nb);
+        }
+        writeback(s2, p0, nb, st, 0, 0, N, p0, p1 + nb1);
+        RGF_HIP(hipEventRecord(ov->ev[2 + st], s2));
+        writeback(s, p0, nb, st, 1, p1, p1 + nb1, 0, 0);
+        if (next) panels(s, p1, nb1, st ^ 1);
+    }
+    RGF_HIP(hipStreamWaitEvent(s, ov->ev[2 + (int)((nblk - 1) & 1)], 0));
     RGF_HIP(hipGetLastError());
 }
 
@@ -1146,7 +1311,8 @@ double max_abs(hipStream_t s, int64_t count, const double* re, const double* im,
     template void cgemm<T>(hipStream_t, int64_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>, cd, int, int64_t, \
                            int64_t, int, T*, size_t);                                                         \
     template size_t inverse_workspace_elems<T>(int64_t, int);                                                 \
-    template void invert_inplace<T>(hipStream_t, int64_t, CMat<T>, int, T*, int*, int, KernelTimer*);         \
+    template void invert_inplace<T>(hipStream_t, int64_t, CMat<T>, int, T*, int*, int, KernelTimer*,           \
+                                    const Overlap*);                                                          \
     template void assemble_slice<T>(hipStream_t, int64_t, CMat<T>, const double*, const double*, cd, cd,      \
                                     const CMat<T>*, const DevCSR&);                                           \
     template void fill_zero<T>(hipStream_t, int64_t, int64_t, CMat<T>);                                       \
