@@ -97,28 +97,36 @@ CMat<T> dmat(rgf_context* c, const std::string& name, int64_t rows, int64_t cols
 
 // host complex row-major -> planar device
 void upload(rgf_context* c, const std::vector<cd>& h, int64_t rows, int64_t cols, CMat<double> d) {
+    if (rows <= 0 || cols <= 0) return;
     std::vector<double> re((size_t)rows * cols), im((size_t)rows * cols);
     for (int64_t i = 0; i < rows; ++i)
         for (int64_t j = 0; j < cols; ++j) {
             re[i * cols + j] = h[i * cols + j].real();
             im[i * cols + j] = h[i * cols + j].imag();
         }
-    for (int64_t i = 0; i < rows; ++i) {
-        RGF_HIP(hipMemcpyAsync(d.re + i * d.ld, re.data() + i * cols, cols * sizeof(double), hipMemcpyHostToDevice,
-                               c->stream));
-        RGF_HIP(hipMemcpyAsync(d.im + i * d.ld, im.data() + i * cols, cols * sizeof(double), hipMemcpyHostToDevice,
-                               c->stream));
+    if (d.ld == cols) {
+        RGF_HIP(hipMemcpyAsync(d.re, re.data(), re.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        RGF_HIP(hipMemcpyAsync(d.im, im.data(), im.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    } else {
+        RGF_HIP(hipMemcpy2DAsync(d.re, d.ld * sizeof(double), re.data(), cols * sizeof(double), cols * sizeof(double),
+                                 rows, hipMemcpyHostToDevice, c->stream));
+        RGF_HIP(hipMemcpy2DAsync(d.im, d.ld * sizeof(double), im.data(), cols * sizeof(double), cols * sizeof(double),
+                                 rows, hipMemcpyHostToDevice, c->stream));
     }
     RGF_HIP(hipStreamSynchronize(c->stream));
 }
 
 std::vector<cd> download(rgf_context* c, CMat<double> d, int64_t rows, int64_t cols) {
+    if (rows <= 0 || cols <= 0) return std::vector<cd>();
     std::vector<double> re((size_t)rows * cols), im((size_t)rows * cols);
-    for (int64_t i = 0; i < rows; ++i) {
-        RGF_HIP(hipMemcpyAsync(re.data() + i * cols, d.re + i * d.ld, cols * sizeof(double), hipMemcpyDeviceToHost,
-                               c->stream));
-        RGF_HIP(hipMemcpyAsync(im.data() + i * cols, d.im + i * d.ld, cols * sizeof(double), hipMemcpyDeviceToHost,
-                               c->stream));
+    if (d.ld == cols) {
+        RGF_HIP(hipMemcpyAsync(re.data(), d.re, re.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        RGF_HIP(hipMemcpyAsync(im.data(), d.im, im.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    } else {
+        RGF_HIP(hipMemcpy2DAsync(re.data(), cols * sizeof(double), d.re, d.ld * sizeof(double), cols * sizeof(double),
+                                 rows, hipMemcpyDeviceToHost, c->stream));
+        RGF_HIP(hipMemcpy2DAsync(im.data(), cols * sizeof(double), d.im, d.ld * sizeof(double), cols * sizeof(double),
+                                 rows, hipMemcpyDeviceToHost, c->stream));
     }
     RGF_HIP(hipStreamSynchronize(c->stream));
     std::vector<cd> h((size_t)rows * cols);

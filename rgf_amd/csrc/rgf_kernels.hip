@@ -94,6 +94,8 @@ struct GemmArgs {
     int beta;
     int srlo, srhi, sclo, schi;
     int vec_ok;
+    int nz, kc;       // split-K: nz chunks of kc along K, chunk z writes C + z * zstride
+    int64_t zstride;
     int tiles_n, tiles;  // tile grid of this problem (set at launch)
 };
 
@@ -103,17 +105,25 @@ template <typename T, int BM, int BN, int BK, int WM, int WN, bool USE3M>
 __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, GemmArgs<T> g1) {
     const bool second = (int)blockIdx.x >= g0.tiles;
     const GemmArgs<T>& g = second ? g1 : g0;
-    const int lin = second ? (int)blockIdx.x - g0.tiles : (int)blockIdx.x;
-    const T* __restrict__ Are = g.Are;
-    const T* __restrict__ Aim = g.Aim;
+    int lin = second ? (int)blockIdx.x - g0.tiles : (int)blockIdx.x;
+    int z = 0;
+    if (g.nz > 1) {
+        const int per = g.tiles / g.nz;
+        z = lin / per;
+        lin -= z * per;
+    }
+    const int64_t koff = (int64_t)z * g.kc;
     const int64_t lda = g.lda;
-    const T* __restrict__ Bre = g.Bre;
-    const T* __restrict__ Bim = g.Bim;
     const int64_t ldb = g.ldb;
-    T* Cre = g.Cre;
-    T* Cim = g.Cim;
+    const T* __restrict__ Are = g.Are + koff;
+    const T* __restrict__ Aim = g.Aim + koff;
+    const T* __restrict__ Bre = g.Bre + koff * ldb;
+    const T* __restrict__ Bim = g.Bim + koff * ldb;
+    T* Cre = g.Cre + z * g.zstride;
+    T* Cim = g.Cim + z * g.zstride;
     const int64_t ldc = g.ldc;
-    const int M = g.M, N = g.N, K = g.K;
+    const int M = g.M, N = g.N;
+    const int K = g.nz > 1 ? (int)min((int64_t)g.kc, (int64_t)g.K - koff) : g.K;
     const double alpha_re = g.alpha_re, alpha_im = g.alpha_im;
     const int beta = g.beta;
     const int srlo = g.srlo, srhi = g.srhi, sclo = g.sclo, schi = g.schi;
@@ -422,6 +432,9 @@ static GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T>
     g.srhi = (int)sk.rhi;
     g.sclo = (int)sk.clo;
     g.schi = (int)sk.chi;
+    g.nz = 1;
+    g.kc = (int)K;
+    g.zstride = 0;
     g.vec_ok = (A.ld % VW == 0) && (B.ld % VW == 0) && ((uintptr_t)A.re % 16 == 0) && ((uintptr_t)A.im % 16 == 0) &&
                ((uintptr_t)B.re % 16 == 0) && ((uintptr_t)B.im % 16 == 0);
     return g;
@@ -432,7 +445,7 @@ static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>
     GemmArgs<T> g0 = g0_, g1 = g1_ ? *g1_ : g0_;
     auto tiles = [](GemmArgs<T>& g) {
         g.tiles_n = (g.N + BN - 1) / BN;
-        g.tiles = g.tiles_n * ((g.M + BM - 1) / BM);
+        g.tiles = g.tiles_n * ((g.M + BM - 1) / BM) * g.nz;
         if (g.M <= 0 || g.N <= 0) g.tiles = 0;
     };
     tiles(g0);
@@ -488,20 +501,18 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
         const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
         if (tiles < 512) {
             const int64_t per = 2 * M * N;  // elements of one partial (two planes)
-            int64_t nz = std::min<int64_t>((1024 + tiles - 1) / tiles, K / 256);
+            int64_t nz = std::min<int64_t>((1536 + tiles - 1) / tiles, K / 256);
             nz = std::min<int64_t>(nz, (int64_t)(ws_elems / per));
             if (nz >= 2) {
                 constexpr int BK = GemmCfg<T>::BK_SMALL;
                 const int64_t kc = ((K + nz - 1) / nz + BK - 1) / BK * BK;
                 nz = (K + kc - 1) / kc;
-                for (int64_t z = 0; z < nz; ++z) {
-                    const int64_t k0 = z * kc, kl = std::min<int64_t>(kc, K - k0);
-                    CMat<T> Az(A.re + k0, A.im + k0, A.ld);
-                    CMat<T> Bz(B.re + k0 * B.ld, B.im + k0 * B.ld, B.ld);
-                    CMat<T> Pz(ws + z * per, ws + z * per + M * N, N);
-                    launch_cgemm<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, M, N, kl, Az, Bz, Pz, cd(1, 0), 0, Skip{},
-                                                                        use3m);
-                }
+                // all chunks in one launch; chunk z writes partial z of the workspace
+                GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, CMat<T>(ws, ws + M * N, N), cd(1, 0), 0, Skip{});
+                g.nz = (int)nz;
+                g.kc = (int)kc;
+                g.zstride = per;
+                launch_pair<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, g, nullptr, use3m);
                 hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
                                    (const T*)ws, per, C.re, C.im, C.ld, alpha.real(), alpha.imag(), beta);
                 RGF_HIP(hipGetLastError());
