@@ -43,6 +43,9 @@ struct Mfma<double> {
     }
     // accumulator element r of lane l holds row (l/16) + 4r
     static __device__ __forceinline__ int row(int l4, int r) { return l4 + 4 * r; }
+    // lane group and element holding row k
+    static constexpr __device__ __forceinline__ int grp(int k) { return k % 4; }
+    static constexpr __device__ __forceinline__ int elt(int k) { return k / 4; }
 };
 
 template <>
@@ -53,6 +56,9 @@ struct Mfma<float> {
     }
     // accumulator element r of lane l holds row 4(l/16) + r
     static __device__ __forceinline__ int row(int l4, int r) { return 4 * l4 + r; }
+    // lane group and element holding row k
+    static constexpr __device__ __forceinline__ int grp(int k) { return k / 4; }
+    static constexpr __device__ __forceinline__ int elt(int k) { return k % 4; }
 };
 
 // ------------------------------------------------------------------ K3: complex GEMM
@@ -590,13 +596,77 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// In-wavefront Gauss-Jordan inverse of a 16x16 complex tile held in the
+// MFMA accumulator layout (lane (g, c) = (l/16, l%16) holds column c, rows
+// Mfma<T>::row(g, r)), without row interchanges.  Step k: the pivot row is
+// fetched with ds_bpermute from lane group grp(k), the multipliers M[i][k]
+// from lane c = k of the own group, the pivot itself with readlane.
+// Returns true when a pivot is zero or not finite.
+template <typename T>
+__device__ __forceinline__ bool gj_tile16(typename Mfma<T>::v4& xr, typename Mfma<T>::v4& xi, int g, int c) {
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int gk = Mfma<T>::grp(k), rk = Mfma<T>::elt(k);
+        const T prr = __shfl(xr[rk], 16 * gk + c);
+        const T pri = __shfl(xi[rk], 16 * gk + c);
+        T fr[4], fi[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            fr[r] = __shfl(xr[r], 16 * g + k);
+            fi[r] = __shfl(xi[r], 16 * g + k);
+        }
+        const T ar = lane_bcast<T>(xr[rk], 16 * gk + k);
+        const T ai = lane_bcast<T>(xi[rk], 16 * gk + k);
+        const T den = ar * ar + ai * ai;
+        if (!(den > T(0)) || !isfinite(den)) bad = true;
+        const T ivr = ar / den, ivi = -ai / den;
+        const bool ck = c == k;
+        const T scr = ck ? ivr : prr * ivr - pri * ivi;  // pivot row / pivot
+        const T sci = ck ? ivi : prr * ivi + pri * ivr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const T gr = fr[r] * ivr - fi[r] * ivi, gi = fr[r] * ivi + fi[r] * ivr;  // M[i][k] / pivot
+            T nr = ck ? -gr : xr[r] - (fr[r] * scr - fi[r] * sci);
+            T ni = ck ? -gi : xi[r] - (fr[r] * sci + fi[r] * scr);
+            if (r == rk) {
+                const bool pr = g == gk;  // this element is in the pivot row
+                nr = pr ? scr : nr;
+                ni = pr ? sci : ni;
+            }
+            xr[r] = nr;
+            xi[r] = ni;
+        }
+    }
+    return bad;
+}
+
+// acc += sgn * A * B for 16x16 complex tiles: A row-major in LDS (row stride
+// ld, planes Ar/Ai), B in registers in the accumulator layout.  The K index
+// of step s at lane group g is Mfma<T>::row(g, s), which makes the B tile's
+// own element s the right operand.
+template <typename T>
+__device__ __forceinline__ void mma_tile16(typename Mfma<T>::v4& accr, typename Mfma<T>::v4& acci, const T* Ar,
+                                           const T* Ai, int ld, const typename Mfma<T>::v4& br,
+                                           const typename Mfma<T>::v4& bi, T sgn, int g, int c) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int k = Mfma<T>::row(g, s);
+        const T ar = sgn * Ar[c * ld + k], ai = sgn * Ai[c * ld + k];
+        accr = Mfma<T>::mma(ar, br[s], accr);
+        acci = Mfma<T>::mma(ar, bi[s], acci);
+        accr = Mfma<T>::mma(-ai, bi[s], accr);
+        acci = Mfma<T>::mma(ai, br[s], acci);
+    }
+}
+
 template <typename T, int NB>
-__global__ __launch_bounds__(512) void gj_diag_blk_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb, T* Dre,
-                                                          T* Dim, int64_t ldd, int* status) {
+__global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb,
+                                                                   T* Dre, T* Dim, int64_t ldd, int* status) {
     using V = typename Mfma<T>::v4;
-    constexpr int NT = 512;
-    constexpr int PW = 16;        // panel width
-    constexpr int RPL = NB / 64;  // rows per lane in wavefront 0
+    constexpr int NW = NB / 16;  // wavefronts; wavefront w owns block column w
+    constexpr int NT = NW * 64;
+    constexpr int PW = 16;       // panel width
     constexpr int LDP = NB + 1;
     constexpr int NBLK = NB / 16;
     static_assert(NB % 64 == 0, "NB must be a multiple of 64");
@@ -609,9 +679,151 @@ __global__ __launch_bounds__(512) void gj_diag_blk_kernel(T* Are, T* Aim, int64_
     __shared__ int bad_s, any_swap;
     __shared__ T colr[2][NB], coli[2][NB], rowr[2][PW], rowi[2][PW], pvr[PW], pvi[PW];
     __shared__ float gstat[3][NT / 64];
+    __shared__ int okflag[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+
+    // ---- fast path: no row interchanges, the matrix in registers.  Panel p
+    // (block column p) belongs to wavefront p: it publishes the column panel
+    // A[:, P] and D = A_PP^-1 through LDS, then every wavefront w != p forms
+    // R = D A[P, w] and A[I, w] -= A[I, P] R (I = rows outside P), and
+    // wavefront p replaces its column by -A[I, P] D and D.  If a panel fails
+    // the growth check (max|D|^2 max|A[:, P]|^2 > 1e6) or meets a zero pivot,
+    // the block is factored again below with partial pivoting.
+    {
+        const int g = lane >> 4, c = lane & 15;
+        // the block (identity padding outside nb) through LDS: coalesced
+        // global loads, then each wavefront takes its block column
+        for (int e = tid; e < NB * NB; e += NT) {
+            const int i = e / NB, j = e % NB;
+            T vr = (i == j) ? T(1) : T(0), vi = 0;
+            if (i < nb && j < nb) {
+                vr = Are[(int64_t)(p0 + i) * lda + p0 + j];
+                vi = Aim[(int64_t)(p0 + i) * lda + p0 + j];
+            }
+            sr[i][j] = vr;
+            si[i][j] = vi;
+        }
+        __syncthreads();
+        V xr[NW], xi[NW];
+#pragma unroll
+        for (int bi = 0; bi < NW; ++bi)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                xr[bi][r] = sr[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c];
+                xi[bi][r] = si[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c];
+            }
+        __syncthreads();
+        // LDS views (aliasing sr / si): column panel [2][NB][CLD], D [2][16][CLD]
+        constexpr int CLD = 17;
+        static_assert(2 * NB * CLD + 2 * 16 * CLD <= NB * LDP, "fast-path buffers exceed LDS");
+        T* const fr = &sr[0][0];
+        T* const fi = &si[0][0];
+        bool ok = true;
+        V dr, di;
+        for (int p = 0; p < NW; ++p) {
+            const int buf = p & 1;
+            T* const cbr = fr + buf * NB * CLD;
+            T* const cbi = fi + buf * NB * CLD;
+            T* const dbr = fr + 2 * NB * CLD + buf * 16 * CLD;
+            T* const dbi = fi + 2 * NB * CLD + buf * 16 * CLD;
+            if (wave == p) {
+                float ma = 0.f;
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = bi * 16 + Mfma<T>::row(g, r);
+                        cbr[i * CLD + c] = xr[bi][r];
+                        cbi[i * CLD + c] = xi[bi][r];
+                        ma = fmaxf(ma, (float)(xr[bi][r] * xr[bi][r] + xi[bi][r] * xi[bi][r]));
+                    }
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi)
+                    if (bi == p) {
+                        dr = xr[bi];
+                        di = xi[bi];
+                    }
+                const bool bad = gj_tile16<T>(dr, di, g, c);
+                float md = 0.f;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    dbr[Mfma<T>::row(g, r) * CLD + c] = dr[r];
+                    dbi[Mfma<T>::row(g, r) * CLD + c] = di[r];
+                    md = fmaxf(md, (float)(dr[r] * dr[r] + di[r] * di[r]));
+                }
+                const float wa = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(ma)));
+                const float wd = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(md)));
+                const float gr = wa * wd;
+                if (lane == 0) okflag[buf] = (!bad && isfinite(gr) && gr <= 1e6f) ? 1 : 0;
+            }
+            __syncthreads();
+            if (!okflag[buf]) {
+                ok = false;
+                break;
+            }
+            if (wave == p) {
+                // own column: -A[I, P] D, and D in the pivot rows
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi) {
+                    if (bi == p) {
+                        xr[bi] = dr;
+                        xi[bi] = di;
+                    } else {
+                        V nr = V{0, 0, 0, 0}, ni = V{0, 0, 0, 0};
+                        mma_tile16<T>(nr, ni, cbr + bi * 16 * CLD, cbi + bi * 16 * CLD, CLD, dr, di, T(-1), g, c);
+                        xr[bi] = nr;
+                        xi[bi] = ni;
+                    }
+                }
+            } else {
+                V br = xr[0], bim = xi[0];
+#pragma unroll
+                for (int bi = 1; bi < NW; ++bi)
+                    if (bi == p) {
+                        br = xr[bi];
+                        bim = xi[bi];
+                    }
+                V rr = V{0, 0, 0, 0}, ri = V{0, 0, 0, 0};
+                mma_tile16<T>(rr, ri, dbr, dbi, CLD, br, bim, T(1), g, c);  // R = D A[P, w]
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi) {
+                    if (bi == p) {
+                        xr[bi] = rr;
+                        xi[bi] = ri;
+                    } else {
+                        mma_tile16<T>(xr[bi], xi[bi], cbr + bi * 16 * CLD, cbi + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
+                                      c);
+                    }
+                }
+            }
+        }
+        if (ok) {
+            __syncthreads();  // the last panel's LDS buffers alias sr / si
+#pragma unroll
+            for (int bi = 0; bi < NW; ++bi)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    sr[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c] = xr[bi][r];
+                    si[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c] = xi[bi][r];
+                }
+            __syncthreads();
+            for (int e = tid; e < NB * NB; e += NT) {
+                const int i = e / NB, j = e % NB;
+                if (i < nb && j < nb) {
+                    Are[(int64_t)(p0 + i) * lda + p0 + j] = sr[i][j];
+                    Aim[(int64_t)(p0 + i) * lda + p0 + j] = si[i][j];
+                    Dre[(int64_t)i * ldd + j] = sr[i][j];
+                    Dim[(int64_t)i * ldd + j] = si[i][j];
+                }
+            }
+            return;
+        }
+    }
+    __syncthreads();
+
+    // ---- fallback: the block again from A, in LDS, with partial pivoting
     const int prow = tid / TPRW, pc0 = (tid % TPRW) * CP;
     if (tid == 0) bad_s = 0;
     for (int e = tid; e < NB * NB; e += NT) {
@@ -924,7 +1136,7 @@ static void launch_diag(hipStream_t s, CMat<T> A, int64_t p0, int nb, CMat<T> D,
             return;
         }
     }
-    hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(512), 0, s, A.re, A.im, A.ld, (int)p0, nb, D.re,
+    hipLaunchKernelGGL((gj_diag_blk_kernel<T, 64>), dim3(1), dim3(256), 0, s, A.re, A.im, A.ld, (int)p0, nb, D.re,
                        D.im, D.ld, st);
 }
 
