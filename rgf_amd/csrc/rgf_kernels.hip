@@ -601,10 +601,14 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 // Mfma<T>::row(g, r)), without row interchanges.  Step k: the pivot row is
 // fetched with ds_bpermute from lane group grp(k), the multipliers M[i][k]
 // from lane c = k of the own group, the pivot itself with readlane.
-// Returns true when a pivot is zero or not finite.
+// Returns true when a pivot is zero or not finite; gmax receives this lane's
+// largest |x|^2 over all intermediate values (the growth of the unpivoted
+// elimination).
 template <typename T>
-__device__ __forceinline__ bool gj_tile16(typename Mfma<T>::v4& xr, typename Mfma<T>::v4& xi, int g, int c) {
+__device__ __forceinline__ bool gj_tile16(typename Mfma<T>::v4& xr, typename Mfma<T>::v4& xi, int g, int c,
+                                          float& gmax) {
     bool bad = false;
+    float gm = 0.f;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const int gk = Mfma<T>::grp(k), rk = Mfma<T>::elt(k);
@@ -636,8 +640,10 @@ __device__ __forceinline__ bool gj_tile16(typename Mfma<T>::v4& xr, typename Mfm
             }
             xr[r] = nr;
             xi[r] = ni;
+            gm = fmaxf(gm, (float)(nr * nr + ni * ni));
         }
     }
+    gmax = gm;
     return bad;
 }
 
@@ -678,7 +684,6 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
     __shared__ int pivs[NB];
     __shared__ int bad_s, any_swap;
     __shared__ T colr[2][NB], coli[2][NB], rowr[2][PW], rowi[2][PW], pvr[PW], pvi[PW];
-    __shared__ float gstat[3][NT / 64];
     __shared__ int okflag[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -689,8 +694,10 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
     // A[:, P] and D = A_PP^-1 through LDS, then every wavefront w != p forms
     // R = D A[P, w] and A[I, w] -= A[I, P] R (I = rows outside P), and
     // wavefront p replaces its column by -A[I, P] D and D.  If a panel fails
-    // the growth check (max|D|^2 max|A[:, P]|^2 > 1e6) or meets a zero pivot,
-    // the block is factored again below with partial pivoting.
+    // the growth checks (max|D| max|A[:, P]|, and the largest intermediate
+    // value of the tile elimination over max|A_PP|, above 31.6 in FP32 or
+    // 100 in FP64) or meets a zero pivot, the block is factored again below with
+    // partial pivoting.
     {
         const int g = lane >> 4, c = lane & 15;
         // the block (identity padding outside nb) through LDS: coalesced
@@ -745,7 +752,11 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                         dr = xr[bi];
                         di = xi[bi];
                     }
-                const bool bad = gj_tile16<T>(dr, di, g, c);
+                float mt = 0.f;  // largest |x|^2 of the pivot tile before elimination
+#pragma unroll
+                for (int r = 0; r < 4; ++r) mt = fmaxf(mt, (float)(dr[r] * dr[r] + di[r] * di[r]));
+                float mg;
+                const bool bad = gj_tile16<T>(dr, di, g, c, mg);
                 float md = 0.f;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -755,8 +766,15 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                 }
                 const float wa = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(ma)));
                 const float wd = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(md)));
+                const float wt = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mt)));
+                const float wg = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mg)));
+                // bounds, squared: multipliers max|A[:, P]| max|D| and the growth
+                // of the in-tile elimination max|x| / max|A_PP| stay below 31.6
+                // (FP32) or 100 (FP64)
+                const float lim = sizeof(T) == 4 ? 1e3f : 1e4f;
                 const float gr = wa * wd;
-                if (lane == 0) okflag[buf] = (!bad && isfinite(gr) && gr <= 1e6f) ? 1 : 0;
+                const bool grow_ok = isfinite(wg) && wg <= lim * fmaxf(wt, 1e-30f);
+                if (lane == 0) okflag[buf] = (!bad && grow_ok && isfinite(gr) && gr <= lim) ? 1 : 0;
             }
             __syncthreads();
             if (!okflag[buf]) {
@@ -838,140 +856,100 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
     }
     __syncthreads();
     for (int q0 = 0; q0 < NB; q0 += PW) {
-        // ---- phase 1: all threads factor the panel [q0, q0+16): thread t
-        // owns CP panel columns of row t / TPRW.  Each pivot step publishes
-        // the pivot row and the pivot column through LDS (double-buffered,
-        // one barrier per step).  The first attempt uses no row interchanges
-        // (the slice matrices are close to diagonally dominant); if the panel
-        // grows by more than 1e3 it is factored again with partial pivoting.
+        // ---- phase 1: all threads factor the panel [q0, q0+16) with partial
+        // pivoting over the rows >= the pivot row: thread t owns CP panel
+        // columns of row t / TPRW.  Each pivot step publishes the pivot row
+        // and the pivot column through LDS (double-buffered, one barrier per
+        // step, one more when rows are interchanged).
         {
             T xr[CP], xi[CP];
-            bool pivoting = false;
+#pragma unroll
+            for (int c = 0; c < CP; ++c) {
+                xr[c] = sr[prow][q0 + pc0 + c];
+                xi[c] = si[prow][q0 + pc0 + c];
+            }
+            int bad = 0;
             bool swapped = false;
-#pragma unroll 1
-            for (int attempt = 0; attempt < 2; ++attempt) {
-                float m0 = 0.f;
 #pragma unroll
-                for (int c = 0; c < CP; ++c) {
-                    xr[c] = sr[prow][q0 + pc0 + c];
-                    xi[c] = si[prow][q0 + pc0 + c];
-                    m0 = fmaxf(m0, (float)(xr[c] * xr[c] + xi[c] * xi[c]));
+            for (int k = 0; k < PW; ++k) {
+                const int kc = q0 + k, bsel = k & 1;
+                // publish column k (its owner in every row) and row kc
+                if ((tid % TPRW) == k / CP) {
+                    colr[bsel][prow] = xr[k % CP];
+                    coli[bsel][prow] = xi[k % CP];
                 }
-                int bad = 0;
-                swapped = false;
+                if (prow == kc) {
 #pragma unroll
-                for (int k = 0; k < PW; ++k) {
-                    const int kc = q0 + k, bsel = k & 1;
-                    // publish column k (its owner in every row) and row kc
-                    if ((tid % TPRW) == k / CP) {
-                        colr[bsel][prow] = xr[k % CP];
-                        coli[bsel][prow] = xi[k % CP];
+                    for (int c = 0; c < CP; ++c) {
+                        rowr[bsel][pc0 + c] = xr[c];
+                        rowi[bsel][pc0 + c] = xi[c];
                     }
-                    if (prow == kc) {
+                }
+                __syncthreads();
+                // every wavefront finds the same pivot: largest |x| among rows >= kc
+                unsigned long long key = 0;
+                for (int rr2 = kc + lane; rr2 < NB; rr2 += 64) {
+                    const T a = colr[bsel][rr2], b = coli[bsel][rr2];
+                    const float m = (float)(a * a + b * b);
+                    const unsigned long long kk = ((unsigned long long)__float_as_uint(m) << 32) |
+                                                  (unsigned long long)(0xffffffffu - (unsigned)rr2);
+                    if (kk > key) key = kk;
+                }
+                key = wave_max_u64(key);
+                const int piv = (int)(0xffffffffu - (unsigned)(key & 0xffffffffu));
+                const bool xch = piv != kc;  // uniform
+                if (xch) {
+                    swapped = true;
+                    if (prow == piv) {
 #pragma unroll
                         for (int c = 0; c < CP; ++c) {
-                            rowr[bsel][pc0 + c] = xr[c];
-                            rowi[bsel][pc0 + c] = xi[c];
+                            pvr[pc0 + c] = xr[c];
+                            pvi[pc0 + c] = xi[c];
                         }
                     }
                     __syncthreads();
-                    int piv = kc;
-                    if (pivoting) {
-                        // every wavefront finds the same pivot: largest |x| among rows >= kc
-                        unsigned long long key = 0;
-                        for (int rr2 = kc + lane; rr2 < NB; rr2 += 64) {
-                            const T a = colr[bsel][rr2], b = coli[bsel][rr2];
-                            const float m = (float)(a * a + b * b);
-                            const unsigned long long kk = ((unsigned long long)__float_as_uint(m) << 32) |
-                                                          (unsigned long long)(0xffffffffu - (unsigned)rr2);
-                            if (kk > key) key = kk;
-                        }
-                        key = wave_max_u64(key);
-                        piv = (int)(0xffffffffu - (unsigned)(key & 0xffffffffu));
-                        if (piv != kc) {  // uniform
-                            swapped = true;
-                            if (prow == piv) {
+                }
+                const T pkr = xch ? pvr[k] : rowr[bsel][k];
+                const T pki = xch ? pvi[k] : rowi[bsel][k];
+                const T den = pkr * pkr + pki * pki;
+                if (!(den > 0) || !isfinite(den)) bad = 1;
+                const T ivr = pkr / den, ivi = -pki / den;
+                if (prow == kc) {
 #pragma unroll
-                                for (int c = 0; c < CP; ++c) {
-                                    pvr[pc0 + c] = xr[c];
-                                    pvi[pc0 + c] = xi[c];
-                                }
-                            }
-                            __syncthreads();
+                    for (int c = 0; c < CP; ++c) {
+                        const int j = pc0 + c;
+                        const T br = xch ? pvr[j] : rowr[bsel][j], bi2 = xch ? pvi[j] : rowi[bsel][j];
+                        xr[c] = (j == k) ? ivr : br * ivr - bi2 * ivi;
+                        xi[c] = (j == k) ? ivi : br * ivi + bi2 * ivr;
+                    }
+                } else {
+                    // row piv receives the old row kc before its elimination
+                    const bool moved = xch && prow == piv;
+                    const T fr = moved ? colr[bsel][kc] : colr[bsel][prow];
+                    const T fi = moved ? coli[bsel][kc] : coli[bsel][prow];
+                    const T gr = fr * ivr - fi * ivi, gi = fr * ivi + fi * ivr;  // f / pivot
+#pragma unroll
+                    for (int c = 0; c < CP; ++c) {
+                        const int j = pc0 + c;
+                        T br = moved ? rowr[bsel][j] : xr[c];
+                        T bi2 = moved ? rowi[bsel][j] : xi[c];
+                        const T yr = xch ? pvr[j] : rowr[bsel][j], yi = xch ? pvi[j] : rowi[bsel][j];
+                        if (j == k) {
+                            xr[c] = -gr;
+                            xi[c] = -gi;
+                        } else {
+                            xr[c] = br - (gr * yr - gi * yi);
+                            xi[c] = bi2 - (gr * yi + gi * yr);
                         }
                     }
-                    const bool xch = pivoting && piv != kc;
-                    const T pkr = xch ? pvr[k] : rowr[bsel][k];
-                    const T pki = xch ? pvi[k] : rowi[bsel][k];
-                    const T den = pkr * pkr + pki * pki;
-                    if (!(den > 0) || !isfinite(den)) bad = 1;
-                    const T ivr = pkr / den, ivi = -pki / den;
-                    if (prow == kc) {
-#pragma unroll
-                        for (int c = 0; c < CP; ++c) {
-                            const int j = pc0 + c;
-                            const T br = xch ? pvr[j] : rowr[bsel][j], bi2 = xch ? pvi[j] : rowi[bsel][j];
-                            xr[c] = (j == k) ? ivr : br * ivr - bi2 * ivi;
-                            xi[c] = (j == k) ? ivi : br * ivi + bi2 * ivr;
-                        }
-                    } else {
-                        // row piv receives the old row kc before its elimination
-                        const bool moved = xch && prow == piv;
-                        const T fr = moved ? colr[bsel][kc] : colr[bsel][prow];
-                        const T fi = moved ? coli[bsel][kc] : coli[bsel][prow];
-                        const T gr = fr * ivr - fi * ivi, gi = fr * ivi + fi * ivr;  // f / pivot
-#pragma unroll
-                        for (int c = 0; c < CP; ++c) {
-                            const int j = pc0 + c;
-                            T br = moved ? rowr[bsel][j] : xr[c];
-                            T bi2 = moved ? rowi[bsel][j] : xi[c];
-                            const T yr = xch ? pvr[j] : rowr[bsel][j], yi = xch ? pvi[j] : rowi[bsel][j];
-                            if (j == k) {
-                                xr[c] = -gr;
-                                xi[c] = -gi;
-                            } else {
-                                xr[c] = br - (gr * yr - gi * yi);
-                                xi[c] = bi2 - (gr * yi + gi * yr);
-                            }
-                        }
-                    }
-                    if (tid == 0) pivs[kc] = piv;
                 }
-                if (pivoting) {
-                    if (bad) bad_s = 1;
-                    break;
-                }
-                // growth check of the unpivoted attempt
-                float m1 = 0.f;
-#pragma unroll
-                for (int c = 0; c < CP; ++c) m1 = fmaxf(m1, (float)(xr[c] * xr[c] + xi[c] * xi[c]));
-                const unsigned w1 = (unsigned)wave_max_u64(__float_as_uint(m1));
-                const unsigned w0 = (unsigned)wave_max_u64(__float_as_uint(m0));
-                const unsigned wb = (unsigned)wave_max_u64((unsigned)bad);
-                if (lane == 0) {
-                    gstat[0][wave] = __uint_as_float(w0);
-                    gstat[1][wave] = __uint_as_float(w1);
-                    gstat[2][wave] = (float)wb;
-                }
-                __syncthreads();
-                float g0 = 0.f, g1 = 0.f, gb = 0.f;
-#pragma unroll
-                for (int w = 0; w < NT / 64; ++w) {
-                    g0 = fmaxf(g0, gstat[0][w]);
-                    g1 = fmaxf(g1, gstat[1][w]);
-                    gb = fmaxf(gb, gstat[2][w]);
-                }
-                __syncthreads();
-                if (gb == 0.f && isfinite(g1) && g1 <= 1e6f * fmaxf(g0, 1e-30f)) break;
-                pivoting = true;
+                if (tid == 0) pivs[kc] = piv;
             }
+            if (bad) bad_s = 1;
 #pragma unroll
             for (int c = 0; c < CP; ++c) {
                 sr[prow][q0 + pc0 + c] = xr[c];
                 si[prow][q0 + pc0 + c] = xi[c];
-            }
-            if (!pivoting) {
-                // nothing to interchange in phase 2a
             }
             if (tid == 0) any_swap = swapped ? 1 : 0;
         }

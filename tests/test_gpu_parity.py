@@ -68,6 +68,24 @@ def test_slice_inverse_pivots_inside_block():
     assert np.abs(X - np.linalg.inv(A)).max() < 1e-12
 
 
+@pytest.mark.parametrize("precision,n,b,tol", [("fp64", 96, 64, 1e-12), ("fp64", 200, 0, 1e-12),
+                                               ("fp32_refined", 256, 128, 2e-5), ("fp32_refined", 150, 64, 2e-5)])
+@pytest.mark.parametrize("where", ["first", "later", "small"])
+def test_slice_inverse_fallback(precision, n, b, tol, where):
+    # The unpivoted register path must hand the block to the pivoting path:
+    # a zero pivot in the first panel, a zero pivot in a later panel, or a
+    # tiny pivot whose multipliers exceed the growth bound.
+    rng = np.random.default_rng(n + b + len(where))
+    A = 3 * np.eye(n) + 0.2 * _crand(rng, n, n) / np.sqrt(n)
+    k = {"first": 0, "later": 37, "small": 21}[where]
+    A[k, k] = 0.0 if where != "small" else 1e-7
+    A[k, (k + 5) % n] = 2.0
+    A[(k + 5) % n, k] = 1.5
+    X = rgf_amd.dense_inverse(A, precision, b)
+    ref = np.linalg.inv(A)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < tol
+
+
 def _cells(s):
     return [c for c in s]
 
