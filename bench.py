@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=2)
     ap.add_argument("--complex-3m", type=int, default=0, help="3-multiplication complex GEMMs in FP32")
     ap.add_argument("--block", type=int, default=0, help="slice-inverse panel width (0 = default)")
+    ap.add_argument("--single-stream", action="store_true",
+                    help="no look-ahead stream in the slice inverse (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -115,7 +117,8 @@ def main():
     syst, out, inn = spheres_3d()
     L = syst["epsilon"].shape[2]
     N = syst["epsilon"].shape[0] * syst["epsilon"].shape[1]
-    opts = {"profile": 1, "complex_3m": args.complex_3m, "block_size": args.block}
+    opts = {"profile": 1, "complex_3m": args.complex_3m, "block_size": args.block,
+            "single_stream": int(args.single_stream)}
 
     def solve(prec, profile=1):
         o = dict(opts, profile=profile)

@@ -95,7 +95,9 @@ typedef struct {
     int32_t complex_3m;       /* FP32 GEMMs: 1 = 3-multiplication complex products     */
     int32_t profile;          /* 1: time the slice-inverse trailing-update kernel with
                                  HIP events on its stream (fills the kernel_* fields) */
-    int32_t reserved[5];
+    int32_t single_stream;    /* 1: no look-ahead second stream in the slice inverse
+                                 (same kernels, one stream; for counter collection)   */
+    int32_t reserved[4];
 } rgf_options;
 
 typedef struct {

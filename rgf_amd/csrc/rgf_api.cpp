@@ -339,7 +339,7 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
                                    cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<double>(s, N, G, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
-        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer, &c->ov);
+        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer, o.single_stream ? nullptr : &c->ov);
         if (pi) {
             cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             std::swap(X, X2);
@@ -404,7 +404,8 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
                                   cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<float>(s, N, Fn(n), nullptr, nullptr, w.epsR, w.k2, &G0R32, w.A0);
-        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer, &c->ov);
+        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer,
+                              o.single_stream ? nullptr : &c->ov);
     }
     RGF_HIP(hipStreamSynchronize(s));
     w.t_slices = now() - t0;

@@ -400,6 +400,228 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
             }
 }
 
+// Streamlined tile kernel for alpha = +-1, beta in {0, 1}, K a multiple of
+// BK and 16-byte aligned operand rows (the shapes of the slice inverse and
+// the refinement sweeps).  Differences to cgemm_kernel:
+//  * loads are unconditional: rows beyond M and column vectors beyond N read
+//    clamped in-bounds addresses (B rows must hold N rounded up to the vector
+//    width), the epilogue masks them;
+//  * LDS is double-buffered (one barrier per K chunk);
+//  * the K index of MFMA step s at lane group g is 4g + s, so each lane
+//    takes its four A values of a 16-deep K block with one 16-byte LDS read;
+//  * ACC3: three accumulator sets (ArBr, AiBi, ArBi + AiBr), no negated
+//    operands; otherwise two sets with -Ai formed in registers (FP64, where
+//    the accumulators dominate the register budget);
+//  * the four MFMA sweeps of a K step are ordered so that dependent MFMAs
+//    are FM*FN instructions apart;
+//  * the sign of alpha is applied when A is staged into LDS;
+//  * the linear tile index is remapped so that consecutive tiles stay on
+//    one XCD (work groups are dealt round-robin over the 8 XCDs).
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3>
+__global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, int total) {
+    int bid = (int)blockIdx.x;
+    {
+        const int q = total >> 3, rem = total & 7, x = bid & 7, loc = bid >> 3;
+        bid = x * q + min(x, rem) + loc;
+    }
+    const bool second = bid >= g0.tiles;
+    const GemmArgs<T>& g = second ? g1 : g0;
+    int lin = second ? bid - g0.tiles : bid;
+    int z = 0;
+    if (g.nz > 1) {
+        const int per = g.tiles / g.nz;
+        z = lin / per;
+        lin -= z * per;
+    }
+    const int64_t koff = (int64_t)z * g.kc;
+    const int64_t lda = g.lda, ldb = g.ldb, ldc = g.ldc;
+    const T* __restrict__ Are = g.Are + koff;
+    const T* __restrict__ Aim = g.Aim + koff;
+    const T* __restrict__ Bre = g.Bre + koff * ldb;
+    const T* __restrict__ Bim = g.Bim + koff * ldb;
+    T* Cre = g.Cre + z * g.zstride;
+    T* Cim = g.Cim + z * g.zstride;
+    const int M = g.M, N = g.N;
+    const int K = g.nz > 1 ? (int)min((int64_t)g.kc, (int64_t)g.K - koff) : g.K;
+    const int tile_n = (lin % g.tiles_n) * BN;
+    const int tile_m = (lin / g.tiles_n) * BM;
+    if (tile_m >= g.srlo && tile_m + BM <= g.srhi) return;
+    if (tile_n >= g.sclo && tile_n + BN <= g.schi) return;
+
+    using V = typename Mfma<T>::v4;
+    using VT = typename Vec<T>::type;
+    constexpr int VW = Vec<T>::W;
+    constexpr int NT = WM * WN * 64;
+    constexpr int TM = BM / WM, TN = BN / WN;
+    constexpr int FM = TM / 16, FN = TN / 16;
+    constexpr int LA = BK + 4, LB = BN + 4;  // padded LDS rows (16-byte multiples)
+    __shared__ __align__(16) T sA[2][2][BM][LA];
+    __shared__ __align__(16) T sB[2][2][BK][LB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int l16 = lane & 15, g4 = lane >> 4;
+    const T sgn = (T)g.alpha_re;  // +-1
+
+    V rr[FM][FN], ii[ACC3 ? FM : 1][ACC3 ? FN : 1], ri[FM][FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+            rr[a][b] = V{0, 0, 0, 0};
+            ri[a][b] = V{0, 0, 0, 0};
+            if constexpr (ACC3) ii[a][b] = V{0, 0, 0, 0};
+        }
+
+    if (g.beta) {
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+                const int gj = tile_n + wn * TN + b * 16 + l16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(g4, r);
+                    if (gi < M && gj < N) {
+                        rr[a][b][r] = Cre[(int64_t)gi * ldc + gj];
+                        ri[a][b][r] = Cim[(int64_t)gi * ldc + gj];
+                    }
+                }
+            }
+    }
+
+    // global -> register staging of one K chunk; B column vectors are read up
+    // to NVW, N rounded up to the vector width (padding inside the rows)
+    const int NVW = (N + VW - 1) / VW * VW;
+    constexpr int AV = BM * BK / VW, BV = BK * BN / VW;
+    constexpr int AQ = (AV + NT - 1) / NT, BQ = (BV + NT - 1) / NT;
+    VT va_r[AQ], va_i[AQ], vb_r[BQ], vb_i[BQ];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int e = tid + q * NT;
+            if (AV % NT == 0 || e < AV) {
+                const int i = e / (BK / VW), kv = (e % (BK / VW)) * VW;
+                const int64_t o = (int64_t)min(tile_m + i, M - 1) * lda + k0 + kv;
+                va_r[q] = *(const VT*)(Are + o);
+                va_i[q] = *(const VT*)(Aim + o);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int e = tid + q * NT;
+            if (BV % NT == 0 || e < BV) {
+                const int k = e / (BN / VW), nv = (e % (BN / VW)) * VW;
+                const int64_t o = (int64_t)(k0 + k) * ldb + min(tile_n + nv, NVW - VW);
+                vb_r[q] = *(const VT*)(Bre + o);
+                vb_i[q] = *(const VT*)(Bim + o);
+            }
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < AQ; ++q) {
+            const int e = tid + q * NT;
+            if (AV % NT == 0 || e < AV) {
+                const int i = e / (BK / VW), kv = (e % (BK / VW)) * VW;
+                *(VT*)&sA[buf][0][i][kv] = va_r[q] * sgn;
+                *(VT*)&sA[buf][1][i][kv] = va_i[q] * sgn;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int e = tid + q * NT;
+            if (BV % NT == 0 || e < BV) {
+                const int k = e / (BN / VW), nv = (e % (BN / VW)) * VW;
+                *(VT*)&sB[buf][0][k][nv] = vb_r[q];
+                *(VT*)&sB[buf][1][k][nv] = vb_i[q];
+            }
+        }
+    };
+
+    const int nk = K / BK;
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int kc = 0; kc < nk; ++kc) {
+        const int buf = kc & 1;
+        if (kc + 1 < nk) fetch((kc + 1) * BK);
+#pragma unroll
+        for (int kb = 0; kb < BK; kb += 16) {
+            T ar[FM][4], ai[FM][4], br[FN][4], bi[FN][4];
+#pragma unroll
+            for (int a = 0; a < FM; ++a) {
+                const int row = wm * TM + a * 16 + l16;
+#pragma unroll
+                for (int u = 0; u < 4; u += VW) {
+                    const VT xr = *(const VT*)&sA[buf][0][row][kb + 4 * g4 + u];
+                    const VT xi = *(const VT*)&sA[buf][1][row][kb + 4 * g4 + u];
+#pragma unroll
+                    for (int v = 0; v < VW; ++v) {
+                        ar[a][u + v] = xr[v];
+                        ai[a][u + v] = xi[v];
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+                const int col = wn * TN + b * 16 + l16;
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    br[b][s2] = sB[buf][0][kb + 4 * g4 + s2][col];
+                    bi[b][s2] = sB[buf][1][kb + 4 * g4 + s2][col];
+                }
+            }
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) {
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) rr[a][b] = Mfma<T>::mma(ar[a][s2], br[b][s2], rr[a][b]);
+                if constexpr (ACC3) {
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b) ii[a][b] = Mfma<T>::mma(ai[a][s2], bi[b][s2], ii[a][b]);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b) rr[a][b] = Mfma<T>::mma(-ai[a][s2], bi[b][s2], rr[a][b]);
+                }
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) ri[a][b] = Mfma<T>::mma(ar[a][s2], bi[b][s2], ri[a][b]);
+#pragma unroll
+                for (int a = 0; a < FM; ++a)
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) ri[a][b] = Mfma<T>::mma(ai[a][s2], br[b][s2], ri[a][b]);
+            }
+        }
+        if (kc + 1 < nk) stash(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue (the accumulators started from C when beta = 1)
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+            const int gj = tile_n + wn * TN + b * 16 + l16;
+            const bool cok = gj < N && !(gj >= g.sclo && gj < g.schi);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(g4, r);
+                if (cok && gi < M && !(gi >= g.srlo && gi < g.srhi)) {
+                    T vr = rr[a][b][r];
+                    if constexpr (ACC3) vr -= ii[a][b][r];
+                    Cre[(int64_t)gi * ldc + gj] = vr;
+                    Cim[(int64_t)gi * ldc + gj] = ri[a][b][r];
+                }
+            }
+        }
+}
+
 template <typename T>
 struct GemmCfg;
 // FP64: 64x64 tiles, 4 waves (32x32 per wave); 128x128, 8 waves (64x32).
@@ -466,6 +688,34 @@ static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>
         hipLaunchKernelGGL((cgemm_kernel<T, BM, BN, BK, WM, WN, false>), grid, block, 0, s, g0, g1);
 }
 
+// cgemm2_kernel launch for one or two problems.  Returns false (launching
+// nothing) when a problem is outside the kernel's preconditions.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4>
+static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_) {
+    constexpr int VW = Vec<T>::W;
+    auto fits = [](const GemmArgs<T>& g) {
+        const bool unit = g.alpha_im == 0.0 && (g.alpha_re == 1.0 || g.alpha_re == -1.0);
+        const bool kok = g.K > 0 && g.K % BK == 0 && (g.nz <= 1 || g.kc % BK == 0);
+        // B column vectors may reach past N up to the next multiple of VW,
+        // which must stay inside the row (ldb)
+        const int64_t nv = ((int64_t)g.N + VW - 1) / VW * VW;
+        return unit && kok && g.vec_ok && nv <= g.ldb && g.M > 0;
+    };
+    if (!fits(g0_) || (g1_ && !fits(*g1_))) return false;
+    GemmArgs<T> g0 = g0_, g1 = g1_ ? *g1_ : g0_;
+    auto tiles = [](GemmArgs<T>& g) {
+        g.tiles_n = (g.N + BN - 1) / BN;
+        g.tiles = g.tiles_n * ((g.M + BM - 1) / BM) * g.nz;
+    };
+    tiles(g0);
+    tiles(g1);
+    const int64_t nblocks = (int64_t)g0.tiles + (g1_ ? g1.tiles : 0);
+    if (nblocks <= 0) return true;
+    hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3>), dim3((unsigned)nblocks), dim3(WM * WN * 64), 0, s,
+                       g0, g1, (int)nblocks);
+    return true;
+}
+
 template <typename T, int BM, int BN, int BK, int WM, int WN>
 static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C,
                          cd alpha, int beta, Skip sk, int use3m) {
@@ -518,7 +768,8 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
                 g.nz = (int)nz;
                 g.kc = (int)kc;
                 g.zstride = per;
-                launch_pair<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, g, nullptr, use3m);
+                if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g, nullptr))
+                    launch_pair<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, g, nullptr, use3m);
                 hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
                                    (const T*)ws, per, C.re, C.im, C.ld, alpha.real(), alpha.imag(), beta);
                 RGF_HIP(hipGetLastError());
@@ -533,6 +784,17 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
     if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX) throw Error(RGF_ERR_INVALID_ARGUMENT, "GEMM too large");
     const int64_t tiles_big = ((M + 127) / 128) * ((N + 127) / 128);
     const int64_t tiles_mid = ((M + 63) / 64) * ((N + 63) / 64);
+    if (!use3m) {
+        // streamlined kernel: 64x64 tiles (4 waves, K chunk 16, two
+        // accumulator sets) when there are enough tiles, else 32x32
+        const GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, C, alpha, beta, sk);
+        const bool done = tiles_mid >= 256 ? launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g, nullptr)
+                                           : launch_pair2<T, 32, 32, 16, 1, 1, false>(s, g, nullptr);
+        if (done) {
+            RGF_HIP(hipGetLastError());
+            return;
+        }
+    }
     // FP64 runs best on 128x128 tiles (8 waves); FP32 on 64x64 tiles, which
     // keep two or three work groups per CU
     if (sizeof(T) == 8 && tiles_big >= 1024)
@@ -544,10 +806,11 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
     RGF_HIP(hipGetLastError());
 }
 
-// Two small products in one launch on 32x32 tiles.
+// Two small products in one launch.
 template <typename T>
 static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, int use3m) {
-    launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
+    if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g0, &g1))
+        launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
     RGF_HIP(hipGetLastError());
 }
 
