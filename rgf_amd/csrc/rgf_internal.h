@@ -65,7 +65,7 @@ struct KernelTimer {
 // Second stream and events for the look-ahead of the slice inverse.
 struct Overlap {
     hipStream_t s2 = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 // ---- kernel launchers (rgf_kernels.hip) -------------------------------------

@@ -1403,12 +1403,15 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
     }
     CMat<T> D(ws + 2 * set, ws + 2 * set + (size_t)b * b, b);
     const int64_t nblk = (N + b - 1) / b;
-    // diagonal block, then R = D A_P: and C = -A_:P D in one launch
-    auto panels = [&](hipStream_t st, int64_t p0, int nb, int q) {
-        launch_diag<T>(st, A, p0, nb, D, dev_status);
+    // R = D A_P: and C = -A_:P D in one launch (D from the diagonal block)
+    auto panel_products = [&](hipStream_t st, int64_t p0, int nb, int q) {
         const GemmArgs<T> gr = gemm_args<T>(nb, N, nb, D, A.offset(p0, 0), R[q], cd(1, 0), 0, Skip{});
         const GemmArgs<T> gc = gemm_args<T>(N, nb, nb, A.offset(0, p0), D, C[q], cd(-1, 0), 0, Skip{});
         cgemm_pair<T>(st, gr, gc, use3m);
+    };
+    auto panels = [&](hipStream_t st, int64_t p0, int nb, int q) {
+        launch_diag<T>(st, A, p0, nb, D, dev_status);
+        panel_products(st, p0, nb, q);
     };
     auto writeback = [&](hipStream_t st, int64_t p0, int nb, int q, int rows, int64_t clo, int64_t chi, int64_t slo,
                          int64_t shi) {
@@ -1438,12 +1441,15 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
     }
     // Look-ahead.  Per block q (P = block q, Q = block q+1):
     //  first stream : after the bulk update of q-1, update block row and
-    //                 column Q with step q (one launch, runs alone);
+    //                 column Q with step q (one launch, runs alone), then
+    //                 invert block Q at once, so that its single work group is
+    //                 dispatched before the bulk update fills the CUs;
     //  second stream: bulk update of everything outside P and Q, reading the
     //                 old column panel in place, then write the column panel
     //                 back outside P and Q;
     //  first stream : meanwhile write the row panel and the Q rows of the
-    //                 column panel back, and factor block Q.
+    //                 column panel back and form Q's panels.
+    // The diagonal block kernel writes only D (its own buffer) and A_QQ.
     hipStream_t s2 = ov->s2;
     panels(s, 0, (int)std::min<int64_t>(b, N), 0);
     for (int64_t q = 0; q < nblk; ++q) {
@@ -1462,6 +1468,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
             cgemm_pair<T>(s, ga, gb, use3m);
         }
         RGF_HIP(hipEventRecord(ov->ev[st], s));
+        if (next) launch_diag<T>(s, A, p1, nb1, D, dev_status);
         RGF_HIP(hipStreamWaitEvent(s2, ov->ev[st], 0));
         if (timer) timer->begin(s2);
         cgemm_skip<T>(s2, N, N, nb, A.offset(0, p0), R[st], A, cd(-1, 0), 1, Skip{p0, p1 + nb1, p0, p1 + nb1}, use3m,
@@ -1473,7 +1480,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         writeback(s2, p0, nb, st, 0, 0, N, p0, p1 + nb1);
         RGF_HIP(hipEventRecord(ov->ev[2 + st], s2));
         writeback(s, p0, nb, st, 1, p1, p1 + nb1, 0, 0);
-        if (next) panels(s, p1, nb1, st ^ 1);
+        if (next) panel_products(s, p1, nb1, st ^ 1);
     }
     RGF_HIP(hipStreamWaitEvent(s, ov->ev[2 + (int)((nblk - 1) & 1)], 0));
     RGF_HIP(hipGetLastError());
