@@ -322,7 +322,10 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     hipStream_t s = c->stream;
     CMat<double> G = dmat<double>(c, "G", N, N);
     double* ws = dbuf<double>(c, "inv_ws", inverse_workspace_elems<double>(N, b));
-    CMat<double> X = dmat<double>(c, "X", N, pi), X2 = dmat<double>(c, "X2", N, pi);
+    // X rows padded to an even length (16-byte rows for the FP64 GEMMs); the
+    // padding column is never read into a stored result
+    const int64_t pip = (pi + 1) / 2 * 2;
+    CMat<double> X = dmat<double>(c, "X", N, pip), X2 = dmat<double>(c, "X2", N, pip);
     CMat<double> Y = dmat<double>(c, "Y", po, N), Y2 = dmat<double>(c, "Y2", po, N);
     CMat<double> Z = dmat<double>(c, "Z", po, pi);
     auto wk = splitk_ws<double>(c);
@@ -378,7 +381,10 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
 // ---------------------------------------------------------------- FP32 + refinement driver
 void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     const int64_t N = w.N, L = w.L, S = L + 2;
-    const int64_t mL = w.inL.size(), mR = w.inR.size(), m = mL + mR;
+    // right-hand sides: mL left columns, mR right columns, zero columns up to
+    // a multiple of 4 (16-byte rows for the FP32 sweep GEMMs; the padding
+    // solves H psi = 0 and stays zero)
+    const int64_t mL = w.inL.size(), mR = w.inR.size(), m = (mL + mR + 3) / 4 * 4;
     const int b = o.block_size > 0 ? o.block_size : default_block(RGF_PRECISION_FP32_REFINED);
     const int use3m = o.complex_3m;
     hipStream_t s = c->stream;
@@ -436,7 +442,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     for (;; ++it) {
         // r = b - H psi
         residual_stencil(s, N, L, m, psi, psi + S * SE, r, r + S * SE, w.eps_re, w.eps_im, w.epsL, w.epsR, w.k2, w.A0,
-                         w.BL.re, w.BL.im, mL, w.BR.re, w.BR.im, 0, 0);
+                         w.BL.re, w.BL.im, mL, w.BR.re, w.BR.im, mR);
         cgemm<double>(s, N, m, N, w.G0L, P(psi, 0), P(r, 0), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
         cgemm<double>(s, N, m, N, w.G0R, P(psi, L + 1), P(r, L + 1), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
         const double rmax = max_abs(s, S * SE, r, r + S * SE, c->d_scratch);

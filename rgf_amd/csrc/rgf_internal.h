@@ -118,13 +118,15 @@ void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const d
                 const double* g, CMat<double> P1, CMat<double> P2);
 
 // Residual of the block-tridiagonal system for slices 0..L+1 (FP64):
-// r_n = b_n - (A0 - k2 diag(eps_n)) psi_n + psi_{n-1} + psi_{n+1}.  The
-// lead self-energy terms at slices 0 and L+1 are added by the caller.
+// r_n = b_n - (A0 - k2 diag(eps_n)) psi_n + psi_{n-1} + psi_{n+1}, with m
+// columns; b_0 holds B_L in columns [0, mL), b_{L+1} holds B_R in columns
+// [mL, mL+mR), all else is zero.  The lead self-energy terms at slices 0 and
+// L+1 are added by the caller.
 void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const double* psi_re,
                       const double* psi_im, double* r_re, double* r_im, const double* eps_re,
                       const double* eps_im, cd epsL, cd epsR, cd k2, const DevCSR& A0,
                       const double* b0_re, const double* b0_im, int64_t mL, const double* b1_re,
-                      const double* b1_im, int64_t slice_stride, int reversed);
+                      const double* b1_im, int64_t mR);
 
 // Adjoint contraction for one slice: with u_i = k2 * sum_j psi[i][j] phi[j][i]
 // (psi N x m, phi m x N), dT/dRe(eps_i) = 4 Im(u_i) and dT/dIm(eps_i) = 4 Re(u_i).

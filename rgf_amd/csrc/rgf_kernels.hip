@@ -415,12 +415,15 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
 //  * the four MFMA sweeps of a K step are ordered so that dependent MFMAs
 //    are FM*FN instructions apart;
 //  * the sign of alpha is applied when A is staged into LDS;
-//  * the linear tile index is remapped so that consecutive tiles stay on
-//    one XCD (work groups are dealt round-robin over the 8 XCDs).
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3>
+//  * MAP 1: the linear index is remapped so that consecutive tiles stay on
+//    one XCD (work groups are dealt round-robin over the 8 XCDs) and tiles
+//    run column-major in groups of 8 tile rows; MAP 0 keeps the row-major
+//    order with round-robin XCDs.  MAP & 2: raise the wave priority
+//    (s_setprio) around each block of MFMAs.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP>
 __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, int total) {
     int bid = (int)blockIdx.x;
-    {
+    if constexpr ((MAP & 1) != 0) {
         const int q = total >> 3, rem = total & 7, x = bid & 7, loc = bid >> 3;
         bid = x * q + min(x, rem) + loc;
     }
@@ -443,8 +446,18 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     T* Cim = g.Cim + z * g.zstride;
     const int M = g.M, N = g.N;
     const int K = g.nz > 1 ? (int)min((int64_t)g.kc, (int64_t)g.K - koff) : g.K;
-    const int tile_n = (lin % g.tiles_n) * BN;
-    const int tile_m = (lin / g.tiles_n) * BM;
+    int tile_n, tile_m;
+    if constexpr ((MAP & 1) != 0) {
+        // groups of 8 tile rows, column-major inside a group
+        const int tiles_m = g.tiles / g.nz / g.tiles_n;
+        const int grp = lin / (8 * g.tiles_n), first = grp * 8;
+        const int gr = min(8, tiles_m - first), idx = lin - grp * 8 * g.tiles_n;
+        tile_m = (first + idx % gr) * BM;
+        tile_n = (idx / gr) * BN;
+    } else {
+        tile_n = (lin % g.tiles_n) * BN;
+        tile_m = (lin / g.tiles_n) * BM;
+    }
     if (tile_m >= g.srlo && tile_m + BM <= g.srhi) return;
     if (tile_n >= g.sclo && tile_n + BN <= g.schi) return;
 
@@ -571,6 +584,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     bi[b][s2] = sB[buf][1][kb + 4 * g4 + s2][col];
                 }
             }
+            if constexpr (MAP & 2) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int s2 = 0; s2 < 4; ++s2) {
 #pragma unroll
@@ -597,6 +611,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
 #pragma unroll
                     for (int b = 0; b < FN; ++b) ri[a][b] = Mfma<T>::mma(ai[a][s2], br[b][s2], ri[a][b]);
             }
+            if constexpr (MAP & 2) __builtin_amdgcn_s_setprio(0);
         }
         if (kc + 1 < nk) stash(buf ^ 1);
         __syncthreads();
@@ -690,7 +705,7 @@ static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>
 
 // cgemm2_kernel launch for one or two problems.  Returns false (launching
 // nothing) when a problem is outside the kernel's preconditions.
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4, int MAP = 0>
 static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_) {
     constexpr int VW = Vec<T>::W;
     auto fits = [](const GemmArgs<T>& g) {
@@ -711,8 +726,8 @@ static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T
     tiles(g1);
     const int64_t nblocks = (int64_t)g0.tiles + (g1_ ? g1.tiles : 0);
     if (nblocks <= 0) return true;
-    hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3>), dim3((unsigned)nblocks), dim3(WM * WN * 64), 0, s,
-                       g0, g1, (int)nblocks);
+    hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP>), dim3((unsigned)nblocks), dim3(WM * WN * 64),
+                       0, s, g0, g1, (int)nblocks);
     return true;
 }
 
@@ -1661,7 +1676,7 @@ __global__ void residual_kernel(int64_t N, int64_t L, int64_t m, const double* p
                                 double* ri, const double* er, const double* ei, double eLr, double eLi, double eRr,
                                 double eRi, double k2r, double k2i, const int32_t* rowptr, const int32_t* col,
                                 const double* vre, const double* vim, const double* b0r, const double* b0i,
-                                int64_t mL, const double* b1r, const double* b1i) {
+                                int64_t mL, const double* b1r, const double* b1i, int64_t mR) {
     const int64_t S = N * m;
     const int64_t total = (L + 2) * S;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
@@ -1703,9 +1718,9 @@ __global__ void residual_kernel(int64_t N, int64_t L, int64_t m, const double* p
             accr += b0r[i * mL + c];
             acci += b0i[i * mL + c];
         }
-        if (n == L + 1 && c >= mL) {
-            accr += b1r[i * (m - mL) + (c - mL)];
-            acci += b1i[i * (m - mL) + (c - mL)];
+        if (n == L + 1 && c >= mL && c < mL + mR) {
+            accr += b1r[i * mR + (c - mL)];
+            acci += b1i[i * mR + (c - mL)];
         }
         rr[e] = accr;
         ri[e] = acci;
@@ -1715,11 +1730,11 @@ __global__ void residual_kernel(int64_t N, int64_t L, int64_t m, const double* p
 void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const double* psi_re, const double* psi_im,
                       double* r_re, double* r_im, const double* eps_re, const double* eps_im, cd epsL, cd epsR,
                       cd k2, const DevCSR& A0, const double* b0_re, const double* b0_im, int64_t mL,
-                      const double* b1_re, const double* b1_im, int64_t, int) {
+                      const double* b1_re, const double* b1_im, int64_t mR) {
     const int64_t total = (L + 2) * N * m;
     hipLaunchKernelGGL(residual_kernel, dim3(grid_for(total)), dim3(256), 0, s, N, L, m, psi_re, psi_im, r_re, r_im,
                        eps_re, eps_im, epsL.real(), epsL.imag(), epsR.real(), epsR.imag(), k2.real(), k2.imag(),
-                       A0.rowptr, A0.col, A0.vre, A0.vim, b0_re, b0_im, mL, b1_re, b1_im);
+                       A0.rowptr, A0.col, A0.vre, A0.vim, b0_re, b0_im, mL, b1_re, b1_im, mR);
     RGF_HIP(hipGetLastError());
 }
 
