@@ -11,8 +11,13 @@ path against it (max entry) and both unitarity defects max|S^H S - I| are
 reported.  value = 128 * steps / elapsed (whole solves, inputs resident).
 
 roofline: the dominant kernel is the rank-b trailing update of the slice
-inverse (MFMA-bound); its flops per launch are 8 (N-b)^2 b (6 with 3M), its
+inverse (MFMA-bound); its flops per launch are 8 (N-2b)^2 b (6 with 3M), its
 duration is summed from HIP events recorded around each launch on its stream.
+traffic (HBM bytes per launch) and MFMA utilisation of that kernel come from
+the rocprofv3 counter summary profiles/pmc_slice_inverse_n4096.md (FETCH_SIZE
+and WRITE_SIZE scaled by a copy kernel of known size, MfmaUtil).
+The mixed path refines to a relative residual of --refine-tol (1e-10), which
+holds the S error against the FP64 path near 1e-11, inside the metric's 1e-9.
 cpu_baseline: the FP64 oracle (numpy/LAPACK) general-scheme slice update at
 the same N, timed on the host cores for a bounded number of slices.
 """
@@ -74,13 +79,33 @@ def cpu_baseline(syst, n_slices: int = 2):
                       f"at N={N} after 1 warm-up update; {dt:.1f} s"}
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_slice_inverse_n4096.md")
+
+
+def counters(prec: str):
+    """Per-launch HBM traffic and MFMA utilisation of the trailing-update kernel
+    from the committed counter summary (its fenced json block), or None."""
+    try:
+        text = open(PMC_SUMMARY).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)[prec]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def roofline(info: dict, prec: str):
     if not info or not info.get("kernel_launches"):
         return None
     t = info["kernel_seconds"]
     ach = info["kernel_flops"] / t / 1e12
+    pmc = counters(prec)
     return {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-            "frac": round(ach / PEAK_TFLOPS[prec], 4), "traffic": None,
+            "frac": round(ach / PEAK_TFLOPS[prec], 4),
+            "traffic": pmc["traffic_bytes"] if pmc else None,
+            "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes": pmc["algorithmic_bytes"] if pmc else None,
+            "mfma_util_pct": pmc["mfma_util_pct"] if pmc else None,
+            "counters_from": os.path.relpath(PMC_SUMMARY, ROOT) if pmc else None,
             "kernel": f"cgemm trailing update of the slice inverse ({prec})",
             "launches": int(info["kernel_launches"]),
             "avg_us": round(1e6 * t / info["kernel_launches"], 2),
@@ -103,6 +128,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="slice-inverse panel width (0 = default)")
     ap.add_argument("--single-stream", action="store_true",
                     help="no look-ahead stream in the slice inverse (for hardware-counter runs)")
+    ap.add_argument("--refine-tol", type=float, default=1e-10,
+                    help="mixed path: stop the FP64 refinement at this relative residual")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -122,6 +149,8 @@ def main():
 
     def solve(prec, profile=1):
         o = dict(opts, profile=profile)
+        if prec == "fp32_refined":
+            o["refine_tol"] = args.refine_tol
         return rgf_amd.cal_smatrix_RGF_3D(syst, out, inn, precision=prec, device=0, options=o, return_info=True)
 
     print(f"# workload: 3D N={N}, {L} slices, full S; warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -151,10 +180,15 @@ def main():
                                "Bloch random phases, lambda 12 px, full S", "n_transverse": N, "slices": L,
                    "n_out": int(S32.shape[0]), "n_in": int(S32.shape[1]), "parallelism": "single GPU"},
         "roofline": roofline(agg, "fp32"),
+        "effective_tflops_reference_formulation": round(value * 32.0 * float(N) ** 3 / 1e12, 1),
+        "refine_tol": args.refine_tol,
         "refine_iters": infos[-1]["refine_iters"], "refine_residual": infos[-1]["refine_residual"],
         "unitarity_defect_fp32_refined": unitarity(S32),
     }
     if not args.skip_fp64:
+        if args.warmup > 0:
+            solve("fp64", profile=0)
+        torch.cuda.synchronize()
         t1 = time.perf_counter()
         S64, _, i64 = solve("fp64")
         t64 = time.perf_counter() - t1

@@ -12,6 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librgf.so")
+DRIVER = os.path.join(HERE, "rgf_inverse_driver")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["rgf_kernels.hip", "rgf_channels.cpp", "rgf_api.cpp"]
 
@@ -51,6 +53,13 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # host program for hardware-counter runs of the slice inverse
+    cmd = ["gcc", "-O2", "-I", INCLUDE, os.path.join(CSRC, "inverse_driver.c"), "-o", DRIVER + ".tmp", "-L", HERE,
+           "-lrgf", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 
