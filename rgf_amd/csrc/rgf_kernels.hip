@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace rgf {
 
@@ -872,73 +873,105 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
-// In-wavefront Gauss-Jordan inverse of a 16x16 complex tile held in the
-// MFMA accumulator layout (lane (g, c) = (l/16, l%16) holds column c, rows
-// Mfma<T>::row(g, r)), without row interchanges.  Step k: the pivot row is
-// fetched with ds_bpermute from lane group grp(k), the multipliers M[i][k]
-// from lane c = k of the own group, the pivot itself with readlane.
-// Returns true when a pivot is zero or not finite; gmax receives this lane's
-// largest |x|^2 over all intermediate values (the growth of the unpivoted
-// elimination).
-template <typename T>
-__device__ __forceinline__ bool gj_tile16(typename Mfma<T>::v4& xr, typename Mfma<T>::v4& xi, int g, int c,
-                                          float& gmax) {
-    bool bad = false;
-    float gm = 0.f;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int gk = Mfma<T>::grp(k), rk = Mfma<T>::elt(k);
-        const T prr = __shfl(xr[rk], 16 * gk + c);
-        const T pri = __shfl(xi[rk], 16 * gk + c);
-        T fr[4], fi[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            fr[r] = __shfl(xr[r], 16 * g + k);
-            fi[r] = __shfl(xi[r], 16 * g + k);
-        }
-        const T ar = lane_bcast<T>(xr[rk], 16 * gk + k);
-        const T ai = lane_bcast<T>(xi[rk], 16 * gk + k);
-        const T den = ar * ar + ai * ai;
-        if (!(den > T(0)) || !isfinite(den)) bad = true;
-        const T ivr = ar / den, ivi = -ai / den;
-        const bool ck = c == k;
-        const T scr = ck ? ivr : prr * ivr - pri * ivi;  // pivot row / pivot
-        const T sci = ck ? ivi : prr * ivi + pri * ivr;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const T gr = fr[r] * ivr - fi[r] * ivi, gi = fr[r] * ivi + fi[r] * ivr;  // M[i][k] / pivot
-            T nr = ck ? -gr : xr[r] - (fr[r] * scr - fi[r] * sci);
-            T ni = ck ? -gi : xi[r] - (fr[r] * sci + fi[r] * scr);
-            if (r == rk) {
-                const bool pr = g == gk;  // this element is in the pivot row
-                nr = pr ? scr : nr;
-                ni = pr ? sci : ni;
-            }
-            xr[r] = nr;
-            xi[r] = ni;
-            gm = fmaxf(gm, (float)(nr * nr + ni * ni));
-        }
-    }
-    gmax = gm;
-    return bad;
+// Value of lane 16 * (lane / 16) + k (the same lane group, column k) via
+// ds_swizzle in bit mode: source = ((lane & 0x10) | k) within each half wave.
+template <int K>
+__device__ __forceinline__ float bcast16(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x10 | (K << 5)));
+}
+template <int K>
+__device__ __forceinline__ double bcast16(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), 0x10 | (K << 5));
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x10 | (K << 5));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// acc += sgn * A * B for 16x16 complex tiles: A row-major in LDS (row stride
-// ld, planes Ar/Ai), B in registers in the accumulator layout.  The K index
-// of step s at lane group g is Mfma<T>::row(g, s), which makes the B tile's
-// own element s the right operand.
+// 1 / (ar + i ai): FP32 through the hardware reciprocal (the FP32 inverses
+// only precondition the FP64 refinement), FP64 by IEEE division.
+__device__ __forceinline__ void cinv(float ar, float ai, float& ivr, float& ivi) {
+    const float r = __builtin_amdgcn_rcpf(ar * ar + ai * ai);
+    ivr = ar * r;
+    ivi = -ai * r;
+}
+__device__ __forceinline__ void cinv(double ar, double ai, double& ivr, double& ivi) {
+    const double den = ar * ar + ai * ai;
+    ivr = ar / den;
+    ivi = -ai / den;
+}
+
+// One Gauss-Jordan step (pivot k) of the in-wavefront inverse of a 16x16
+// complex tile held in the MFMA accumulator layout (lane (g, c) = (l/16, l%16)
+// holds column c, rows Mfma<T>::row(g, r)), without row interchanges.  The
+// pivot row comes by ds_bpermute from lane group grp(k), the multipliers
+// M[i][k] by ds_swizzle from lane c = k of the own group, the pivot by
+// readlane.  bad is set on a zero or non-finite pivot; gm tracks the largest
+// |x|^2 of the new values (the growth of the unpivoted elimination).
+template <typename T, int K>
+__device__ __forceinline__ void gj_step16(typename Mfma<T>::v4& xr, typename Mfma<T>::v4& xi, int g, int c,
+                                          bool& bad, float& gm) {
+    constexpr int gk = Mfma<T>::grp(K), rk = Mfma<T>::elt(K);
+    const T prr = __shfl(xr[rk], 16 * gk + c);
+    const T pri = __shfl(xi[rk], 16 * gk + c);
+    T fr[4], fi[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        fr[r] = bcast16<K>(xr[r]);
+        fi[r] = bcast16<K>(xi[r]);
+    }
+    const T ar = lane_bcast<T>(xr[rk], 16 * gk + K);
+    const T ai = lane_bcast<T>(xi[rk], 16 * gk + K);
+    const T den = ar * ar + ai * ai;
+    if (!(den > T(0)) || !isfinite(den)) bad = true;
+    T ivr, ivi;
+    cinv(ar, ai, ivr, ivi);
+    const bool ck = c == K;
+    const T scr = ck ? ivr : prr * ivr - pri * ivi;  // pivot row / pivot
+    const T sci = ck ? ivi : prr * ivi + pri * ivr;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const T gr = fr[r] * ivr - fi[r] * ivi, gi = fr[r] * ivi + fi[r] * ivr;  // M[i][k] / pivot
+        T nr = ck ? -gr : xr[r] - (fr[r] * scr - fi[r] * sci);
+        T ni = ck ? -gi : xi[r] - (fr[r] * sci + fi[r] * scr);
+        if (r == rk) {
+            const bool pr = g == gk;  // this element is in the pivot row
+            nr = pr ? scr : nr;
+            ni = pr ? sci : ni;
+        }
+        xr[r] = nr;
+        xi[r] = ni;
+        gm = fmaxf(gm, (float)(nr * nr + ni * ni));
+    }
+}
+
+// MFMA step s of acc += sgn * A * B for 16x16 complex tiles: A row-major in
+// LDS (row stride ld, planes Ar/Ai), B in registers in the accumulator
+// layout.  The K index of step s at lane group g is Mfma<T>::row(g, s), which
+// makes the B tile's own element s the right operand.
+template <typename T>
+__device__ __forceinline__ void mma_step16(typename Mfma<T>::v4& accr, typename Mfma<T>::v4& acci, const T* Ar,
+                                           const T* Ai, int ld, T br, T bi, T sgn, int g, int c, int s) {
+    const int k = Mfma<T>::row(g, s);
+    const T ar = sgn * Ar[c * ld + k], ai = sgn * Ai[c * ld + k];
+    accr = Mfma<T>::mma(ar, br, accr);
+    acci = Mfma<T>::mma(ar, bi, acci);
+    accr = Mfma<T>::mma(-ai, bi, accr);
+    acci = Mfma<T>::mma(ai, br, acci);
+}
 template <typename T>
 __device__ __forceinline__ void mma_tile16(typename Mfma<T>::v4& accr, typename Mfma<T>::v4& acci, const T* Ar,
                                            const T* Ai, int ld, const typename Mfma<T>::v4& br,
                                            const typename Mfma<T>::v4& bi, T sgn, int g, int c) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int k = Mfma<T>::row(g, s);
-        const T ar = sgn * Ar[c * ld + k], ai = sgn * Ai[c * ld + k];
-        accr = Mfma<T>::mma(ar, br[s], accr);
-        acci = Mfma<T>::mma(ar, bi[s], acci);
-        accr = Mfma<T>::mma(-ai, bi[s], accr);
-        acci = Mfma<T>::mma(ai, br[s], acci);
+    for (int s = 0; s < 4; ++s) mma_step16<T>(accr, acci, Ar, Ai, ld, br[s], bi[s], sgn, g, c, s);
+}
+
+// Compile-time loop helper: f(std::integral_constant<int, K>) for K in [0, N).
+template <int K, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (K < N) {
+        f(std::integral_constant<int, K>{});
+        static_for<K + 1, N>(f);
     }
 }
 
@@ -947,6 +980,9 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                                                                    T* Dre, T* Dim, int64_t ldd, int* status) {
     using V = typename Mfma<T>::v4;
     constexpr int NW = NB / 16;  // wavefronts; wavefront w owns block column w
+    // the next owner overlaps its remaining tile updates with the elimination
+    // (needs the register room of one wavefront per SIMD)
+    constexpr bool INTERLEAVE = NW <= 4;
     constexpr int NT = NW * 64;
     constexpr int PW = 16;       // panel width
     constexpr int LDP = NB + 1;
@@ -999,60 +1035,76 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
             }
         __syncthreads();
         // LDS views (aliasing sr / si): column panel [2][NB][CLD], D [2][16][CLD]
-        constexpr int CLD = 17;
+        // (rows of CLD elements, 16-byte multiples, so the four K values an
+        // MFMA lane takes are one aligned read)
+        constexpr int CLD = 20;
         static_assert(2 * NB * CLD + 2 * 16 * CLD <= NB * LDP, "fast-path buffers exceed LDS");
         T* const fr = &sr[0][0];
         T* const fi = &si[0][0];
-        bool ok = true;
-        V dr, di;
-        for (int p = 0; p < NW; ++p) {
-            const int buf = p & 1;
-            T* const cbr = fr + buf * NB * CLD;
-            T* const cbi = fi + buf * NB * CLD;
-            T* const dbr = fr + 2 * NB * CLD + buf * 16 * CLD;
-            T* const dbi = fi + 2 * NB * CLD + buf * 16 * CLD;
-            if (wave == p) {
-                float ma = 0.f;
+        auto cbuf = [&](int buf, int plane) { return (plane ? fi : fr) + buf * NB * CLD; };
+        auto dbuf = [&](int buf, int plane) { return (plane ? fi : fr) + 2 * NB * CLD + buf * 16 * CLD; };
+        // D of the panel this wavefront owns (kept from its factorization)
+        V dr = V{0, 0, 0, 0}, di = V{0, 0, 0, 0};
+        // Factor panel q (called by wavefront q once its block column holds
+        // the values after panel q-1): invert the pivot tile into dr/di while
+        // `other` (if given) brings the remaining tiles up to date, then
+        // publish the column panel and D for panel q and the verdict of the
+        // growth checks.
+        auto factor = [&](int q, auto&& other) {
 #pragma unroll
-                for (int bi = 0; bi < NW; ++bi)
+            for (int bi = 0; bi < NW; ++bi)
+                if (bi == q) {
+                    dr = xr[bi];
+                    di = xi[bi];
+                }
+            float mt = 0.f;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int i = bi * 16 + Mfma<T>::row(g, r);
-                        cbr[i * CLD + c] = xr[bi][r];
-                        cbi[i * CLD + c] = xi[bi][r];
-                        ma = fmaxf(ma, (float)(xr[bi][r] * xr[bi][r] + xi[bi][r] * xi[bi][r]));
-                    }
+            for (int r = 0; r < 4; ++r) mt = fmaxf(mt, (float)(dr[r] * dr[r] + di[r] * di[r]));
+            bool bad = false;
+            float mg = 0.f;
+            static_for<0, 16>([&](auto kc) {
+                gj_step16<T, decltype(kc)::value>(dr, di, g, c, bad, mg);
+                other(decltype(kc)::value);
+            });
+            const int buf = q & 1;
+            T* const cr = cbuf(buf, 0);
+            T* const ci = cbuf(buf, 1);
+            float ma = 0.f;
 #pragma unroll
-                for (int bi = 0; bi < NW; ++bi)
-                    if (bi == p) {
-                        dr = xr[bi];
-                        di = xi[bi];
-                    }
-                float mt = 0.f;  // largest |x|^2 of the pivot tile before elimination
-#pragma unroll
-                for (int r = 0; r < 4; ++r) mt = fmaxf(mt, (float)(dr[r] * dr[r] + di[r] * di[r]));
-                float mg;
-                const bool bad = gj_tile16<T>(dr, di, g, c, mg);
-                float md = 0.f;
+            for (int bi = 0; bi < NW; ++bi)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    dbr[Mfma<T>::row(g, r) * CLD + c] = dr[r];
-                    dbi[Mfma<T>::row(g, r) * CLD + c] = di[r];
-                    md = fmaxf(md, (float)(dr[r] * dr[r] + di[r] * di[r]));
+                    const int i = bi * 16 + Mfma<T>::row(g, r);
+                    cr[i * CLD + c] = xr[bi][r];
+                    ci[i * CLD + c] = xi[bi][r];
+                    ma = fmaxf(ma, (float)(xr[bi][r] * xr[bi][r] + xi[bi][r] * xi[bi][r]));
                 }
-                const float wa = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(ma)));
-                const float wd = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(md)));
-                const float wt = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mt)));
-                const float wg = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mg)));
-                // bounds, squared: multipliers max|A[:, P]| max|D| and the growth
-                // of the in-tile elimination max|x| / max|A_PP| stay below 31.6
-                // (FP32) or 100 (FP64)
-                const float lim = sizeof(T) == 4 ? 1e3f : 1e4f;
-                const float gr = wa * wd;
-                const bool grow_ok = isfinite(wg) && wg <= lim * fmaxf(wt, 1e-30f);
-                if (lane == 0) okflag[buf] = (!bad && grow_ok && isfinite(gr) && gr <= lim) ? 1 : 0;
+            float md = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                dbuf(buf, 0)[Mfma<T>::row(g, r) * CLD + c] = dr[r];
+                dbuf(buf, 1)[Mfma<T>::row(g, r) * CLD + c] = di[r];
+                md = fmaxf(md, (float)(dr[r] * dr[r] + di[r] * di[r]));
             }
-            __syncthreads();
+            const float wa = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(ma)));
+            const float wd = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(md)));
+            const float wt = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mt)));
+            const float wg = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mg)));
+            // bounds, squared: multipliers max|A[:, P]| max|D| and the growth
+            // of the in-tile elimination max|x| / max|A_PP| stay below 31.6
+            // (FP32) or 100 (FP64)
+            const float lim = sizeof(T) == 4 ? 1e3f : 1e4f;
+            const float gr = wa * wd;
+            const bool grow_ok = isfinite(wg) && wg <= lim * fmaxf(wt, 1e-30f);
+            if (lane == 0) okflag[buf] = (!bad && grow_ok && isfinite(gr) && gr <= lim) ? 1 : 0;
+        };
+        if (wave == 0) factor(0, [](int) {});
+        bool ok = true;
+        for (int p = 0; p < NW; ++p) {
+            const int buf = p & 1;
+            const T* const cr = cbuf(buf, 0);
+            const T* const ci = cbuf(buf, 1);
+            __syncthreads();  // panel p's column and D are published
             if (!okflag[buf]) {
                 ok = false;
                 break;
@@ -1066,31 +1118,67 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                         xi[bi] = di;
                     } else {
                         V nr = V{0, 0, 0, 0}, ni = V{0, 0, 0, 0};
-                        mma_tile16<T>(nr, ni, cbr + bi * 16 * CLD, cbi + bi * 16 * CLD, CLD, dr, di, T(-1), g, c);
+                        mma_tile16<T>(nr, ni, cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, dr, di, T(-1), g, c);
                         xr[bi] = nr;
                         xi[bi] = ni;
                     }
                 }
-            } else {
-                V br = xr[0], bim = xi[0];
+                continue;
+            }
+            // R = D A[P, w]; then A[I, w] -= A[I, P] R and A[P, w] = R
+            V br = xr[0], bim = xi[0];
 #pragma unroll
-                for (int bi = 1; bi < NW; ++bi)
-                    if (bi == p) {
-                        br = xr[bi];
-                        bim = xi[bi];
-                    }
-                V rr = V{0, 0, 0, 0}, ri = V{0, 0, 0, 0};
-                mma_tile16<T>(rr, ri, dbr, dbi, CLD, br, bim, T(1), g, c);  // R = D A[P, w]
-#pragma unroll
-                for (int bi = 0; bi < NW; ++bi) {
-                    if (bi == p) {
-                        xr[bi] = rr;
-                        xi[bi] = ri;
-                    } else {
-                        mma_tile16<T>(xr[bi], xi[bi], cbr + bi * 16 * CLD, cbi + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
-                                      c);
-                    }
+            for (int bi = 1; bi < NW; ++bi)
+                if (bi == p) {
+                    br = xr[bi];
+                    bim = xi[bi];
                 }
+            V rr = V{0, 0, 0, 0}, ri = V{0, 0, 0, 0};
+            mma_tile16<T>(rr, ri, dbuf(buf, 0), dbuf(buf, 1), CLD, br, bim, T(1), g, c);
+#pragma unroll
+            for (int bi = 0; bi < NW; ++bi)
+                if (bi == p) {
+                    xr[bi] = rr;
+                    xi[bi] = ri;
+                }
+            if (wave == p + 1 && !INTERLEAVE) {
+                // next panel's owner: bring the column up to date, then factor
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi)
+                    if (bi != p)
+                        mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
+                                      c);
+                factor(p + 1, [](int) {});
+            } else if (wave == p + 1) {
+                // next panel's owner: its pivot tile first, then the
+                // elimination with the other tiles' MFMA steps in between
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi)
+                    if (bi == p + 1)
+                        mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
+                                      c);
+                factor(p + 1, [&](int k) {
+                    // MFMA steps spread over the 16 elimination steps: tile
+                    // bi takes steps 16 bi / NW .. 16 (bi + 1) / NW - 1
+#pragma unroll
+                    for (int bi = 0; bi < NW; ++bi) {
+                        constexpr int per = 16 / NW;  // elimination steps per tile
+                        if (k / per != bi || bi == p || bi == p + 1) continue;
+                        constexpr int sper = 4 / per;  // MFMA steps per elimination step
+#pragma unroll
+                        for (int u = 0; u < sper; ++u) {
+                            const int s4 = (k % per) * sper + u;
+                            mma_step16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr[s4],
+                                          ri[s4], T(-1), g, c, s4);
+                        }
+                    }
+                });
+            } else {
+#pragma unroll
+                for (int bi = 0; bi < NW; ++bi)
+                    if (bi != p)
+                        mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
+                                      c);
             }
         }
         if (ok) {
