@@ -97,7 +97,9 @@ typedef struct {
                                  HIP events on its stream (fills the kernel_* fields) */
     int32_t single_stream;    /* 1: no look-ahead second stream in the slice inverse
                                  (same kernels, one stream; for counter collection)   */
-    int32_t reserved[4];
+    int32_t batch_concurrency;/* rgf_solve_batched: solves in flight at once, each on
+                                 its own streams and workspace; 0 = 8                 */
+    int32_t reserved[3];
 } rgf_options;
 
 typedef struct {
@@ -169,7 +171,11 @@ rgf_status rgf_build_laplacian_1d(int64_t n, int32_t bc, double ka, double* A_ou
 rgf_status rgf_solve(rgf_handle h, const rgf_problem* p, const rgf_selection* sel,
                      const rgf_options* opt, double* S_out, rgf_info* info);
 
-/* Many independent solves (frequencies, Bloch phases, widths) in one call. */
+/* Many independent solves (frequencies, Bloch phases, widths) in one call:
+ * up to opt->batch_concurrency solves run at once, each on its own worker
+ * context (streams and device workspace, kept by the handle for later calls).
+ * Results are identical to rgf_solve of each problem.  On failure the status
+ * of the first failing problem is returned and rgf_last_error names it. */
 rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p,
                              const rgf_selection* sel, const rgf_options* opt,
                              double* const* S_out, rgf_info* info);

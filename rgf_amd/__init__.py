@@ -9,6 +9,7 @@ from .api import (  # noqa: F401
     build_laplacian_1d,
     cal_smatrix_RGF,
     cal_smatrix_RGF_3D,
+    cal_smatrix_RGF_batched,
     dense_gemm,
     dense_inverse,
     setup_channels,
@@ -19,5 +20,5 @@ from .api import (  # noqa: F401
 __all__ = [
     "cal_smatrix_RGF", "cal_smatrix_RGF_3D", "setup_channels", "setup_channels_3D",
     "build_laplacian_1d", "load_library", "RGFError", "dense_inverse", "dense_gemm",
-    "transmission_gradient",
+    "transmission_gradient", "cal_smatrix_RGF_batched",
 ]

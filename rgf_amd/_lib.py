@@ -51,7 +51,7 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("max_refine_iter", C.c_int32), ("refine_tol", C.c_double),
         ("block_size", C.c_int32), ("complex_3m", C.c_int32), ("profile", C.c_int32),
-        ("single_stream", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("single_stream", C.c_int32), ("batch_concurrency", C.c_int32), ("reserved", C.c_int32 * 3),
     ]
 
 

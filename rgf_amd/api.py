@@ -263,6 +263,43 @@ def cal_smatrix_RGF_3D(syst: dict, out, in_, *, precision: str = "fp64", device:
     return (S, ch, info) if return_info else (S, ch)
 
 
+def cal_smatrix_RGF_batched(items, *, precision: str = "fp64", device: int = -1, options: dict | None = None,
+                            return_info: bool = False):
+    """Many independent solves in one call (rgf_solve_batched).
+
+    ``items`` is a sequence of ``(syst, out, in_)``; a syst whose ``epsilon``
+    has three dimensions is a 3D problem (cal_smatrix_RGF_3D), otherwise 2D
+    (cal_smatrix_RGF).  Up to ``options['batch_concurrency']`` (default 8)
+    solves run at once.  Returns the list of S matrices (and the infos).
+    """
+    lib = L.load_library()
+    items = list(items)
+    probs, sels, Ss, keep = [], [], [], []
+    for syst, out, in_ in items:
+        prob = _problem_3d(syst) if np.ndim(syst["epsilon"]) == 3 else _problem_2d(syst)
+        nprop = _query(prob)
+        sel = _Sel(_parse_selection(out, "out", nprop), _parse_selection(in_, "in", nprop))
+        nout, nin = C.c_int32(), C.c_int32()
+        L.check(lib.rgf_query(C.byref(prob.p), C.byref(sel.s), None, None, C.byref(nout), C.byref(nin)))
+        probs.append(prob)
+        sels.append(sel)
+        Ss.append(np.zeros((nout.value, nin.value), dtype=np.complex128, order="F"))
+    n = len(items)
+    if n == 0:
+        return ([], []) if return_info else []
+    P = (L.Problem * n)(*[pr.p for pr in probs])
+    Sel = (L.Selection * n)(*[se.s for se in sels])
+    # an empty request still needs a valid pointer; its S stays as returned by rgf_query (real zeros)
+    dummy = np.zeros(2)
+    outs = (L._dp * n)(*[(S if S.size else dummy).ctypes.data_as(L._dp) for S in Ss])
+    infos = (L.Info * n)()
+    o = _options(precision, **(options or {}))
+    h = L.handle(device)
+    L.check(lib.rgf_solve_batched(h, n, P, Sel, C.byref(o), outs, infos), h)
+    res = [S if S.size else np.zeros(S.shape) for S in Ss]
+    return (res, [i.as_dict() for i in infos]) if return_info else res
+
+
 def _channels_raw(prob: _Prob, N: int):
     lib = L.load_library()
     c = L.Channels()

@@ -29,7 +29,9 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <string>
 
 #include "rgf_host.h"
@@ -50,6 +52,7 @@ struct rgf_context {
     std::map<std::string, DevBuf> bufs;
     int* d_status = nullptr;
     double* d_scratch = nullptr;
+    std::vector<rgf_context*> workers;  // worker contexts of rgf_solve_batched
 };
 
 static std::string g_last_error;
@@ -862,6 +865,8 @@ rgf_status rgf_create(rgf_handle* h, int32_t device) {
 
 rgf_status rgf_destroy(rgf_handle h) {
     if (!h) return RGF_OK;
+    for (auto w : h->workers) (void)rgf_destroy(w);
+    h->workers.clear();
     rgf_status st = guard(nullptr, [&] {
         RGF_HIP(hipSetDevice(h->device));
         RGF_HIP(hipStreamSynchronize(h->stream));
@@ -980,7 +985,53 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
     return guard(h, [&] {
         if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
         if (count < 0 || (count > 0 && (!p || !s || !S_out))) throw Error(RGF_ERR_INVALID_ARGUMENT, "bad batch");
-        for (int32_t q = 0; q < count; ++q) solve_one(h, &p[q], &s[q], opt, S_out[q], info ? &info[q] : nullptr);
+        if (count == 0) return;
+        const int want = opt && opt->batch_concurrency > 0 ? opt->batch_concurrency : 8;
+        const int nw = std::max(1, std::min<int>(want, count));
+        while ((int)h->workers.size() < nw) {
+            rgf_handle w = nullptr;
+            const rgf_status cs = rgf_create(&w, h->device);
+            if (cs != RGF_OK) throw Error(cs, std::string("batched solve: worker context: ") + rgf_last_error(nullptr));
+            h->workers.push_back(w);
+        }
+        // largest problems first (N^2 L): balances the workers and lets each
+        // worker's workspace reach its final size on its first problem
+        std::vector<int32_t> order((size_t)count);
+        std::vector<double> cost((size_t)count);
+        for (int32_t q = 0; q < count; ++q) {
+            order[(size_t)q] = q;
+            const int nd = p[q].ndim == 3 ? 3 : 2;  // invalid ndim is reported by the solve itself
+            double n = 1;
+            for (int d = 0; d < nd - 1; ++d) n *= (double)std::max<int64_t>(p[q].dims[d], 1);
+            cost[(size_t)q] = n * n * (double)std::max<int64_t>(p[q].dims[nd - 1], 1);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+        std::atomic<int32_t> next{0};
+        std::vector<rgf_status> sts((size_t)count, RGF_OK);
+        std::vector<std::string> msgs((size_t)count);
+        auto run = [&](int wi) {
+            rgf_context* c = h->workers[(size_t)wi];
+            if (hipSetDevice(c->device) != hipSuccess) {
+                for (int32_t k = next++; k < count; k = next++) sts[(size_t)order[(size_t)k]] = RGF_ERR_DEVICE;
+                return;
+            }
+            for (int32_t k = next++; k < count; k = next++) {
+                const int32_t q = order[(size_t)k];
+                sts[(size_t)q] = guard(c, [&] { solve_one(c, &p[q], &s[q], opt, S_out[q], info ? &info[q] : nullptr); });
+                if (sts[(size_t)q] != RGF_OK) msgs[(size_t)q] = c->err;
+            }
+        };
+        if (nw == 1) {
+            run(0);
+        } else {
+            std::vector<std::thread> th;
+            th.reserve((size_t)nw);
+            for (int wi = 0; wi < nw; ++wi) th.emplace_back(run, wi);
+            for (auto& t : th) t.join();
+        }
+        for (int32_t q = 0; q < count; ++q)
+            if (sts[(size_t)q] != RGF_OK)
+                throw Error(sts[(size_t)q], "batched solve, problem " + std::to_string(q) + ": " + msgs[(size_t)q]);
     });
 }
 

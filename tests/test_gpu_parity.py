@@ -203,3 +203,66 @@ def test_wide_slab_mixed_vs_fp64():
     assert S64.shape == (208 + 320, 208 + 320)
     assert np.abs(S32 - S64).max() < TOL_MIXED
     assert np.linalg.eigvalsh(S64.conj().T @ S64).max() <= 1 + 1e-9
+
+
+def test_batched_ragged_vs_single_and_oracle():
+    """rgf_solve_batched on a ragged batch (2D widths 24..64, a 3D slab, mixed
+    channel requests, both precisions): every S equals the single-solve S of
+    the same problem and matches the oracle."""
+    rng = np.random.default_rng(11)
+    items = []
+    for k, ny in enumerate([24, 40, 64, 32, 48, 24, 56, 40, 64]):
+        nx = int(rng.integers(10, 40))
+        eps = 1 + 2 * rng.random((ny, nx)) + 0.01j * rng.random((ny, nx))
+        lam = 8 + 12 * rng.random()
+        syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.0 + 0.5 * (k % 2),
+                    k0dx=(2 * np.pi / lam) * (1 + 1e-3j * rng.random()), yBC=["periodic", "Dirichlet", 0.7][k % 3])
+        sel = (["L", "R"], ["L", "R"]) if k % 3 else (["R"], {"ind_in_L": [1, 2]})
+        items.append((syst, *sel))
+    eps3 = 1 + rng.random((5, 6, 7))
+    items.append((dict(epsilon=eps3, epsilon_L=1.0, epsilon_R=1.2, k0dx=1.1, xyBC=[0.3, 1.7]), ["L", "R"], ["L"]))
+    for precision, tol in (("fp64", 1e-10), ("fp32_refined", 1e-9)):
+        Sb = rgf_amd.cal_smatrix_RGF_batched(items, precision=precision, options={"batch_concurrency": 4})
+        for (syst, out, inn), S in zip(items, Sb):
+            three_d = np.ndim(syst["epsilon"]) == 3
+            single = rgf_amd.cal_smatrix_RGF_3D if three_d else rgf_amd.cal_smatrix_RGF
+            S1, _ = single(syst, out, inn, precision=precision)
+            assert S.shape == S1.shape
+            assert np.abs(S - S1).max() <= 1e-12
+            So = (o.cal_smatrix_RGF_3D if three_d else o.cal_smatrix_RGF)(syst, out, inn)[0]
+            assert np.abs(S - So).max() < tol
+
+
+def test_batched_sweep_workload_timing():
+    """The 256-solve sweep workload shape (widths 128..512, complex k0dx), cut
+    to 32 solves of 100 slices: batched results equal single solves, and the
+    batched call is timed against the sequential loop."""
+    import time
+    from rgf_amd.workloads import batched_sweep
+    items = [(dict(s, epsilon=s["epsilon"][:, :100]), out, inn) for s, out, inn in batched_sweep(count=32)]
+    t0 = time.perf_counter()
+    Sb = rgf_amd.cal_smatrix_RGF_batched(items)
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Ss = [rgf_amd.cal_smatrix_RGF(s, out, inn)[0] for s, out, inn in items]
+    ts = time.perf_counter() - t0
+    for a, b in zip(Sb, Ss):
+        assert np.abs(a - b).max() <= 1e-12
+        assert np.linalg.eigvalsh(a.conj().T @ a).max() <= 1 + 1e-9  # absorbing (Im k0dx > 0): passive
+    print(f"batched {tb:.3f} s, sequential {ts:.3f} s, speed-up {ts / tb:.2f}", flush=True)
+
+
+def test_cfg2_full_size_mixed_vs_fp64():
+    """2D N=1024 Dirichlet, 2000 slices, eps = 1 + 1.25 Bernoulli(0.3) + 0.01i,
+    eps_L = 1, eps_R = 2.25, k0dx = 2 pi / 10, full S (528 x 528): the mixed
+    path against the FP64 path, passivity of both."""
+    rng = np.random.default_rng(12)
+    eps = 1 + 1.25 * (rng.random((1024, 2000)) < 0.3) + 0.01j
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=2.25, k0dx=2 * np.pi / 10, yBC="Dirichlet")
+    S64, _ = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"])
+    S32, _, info = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"], precision="fp32_refined", return_info=True)
+    assert S64.shape == (528, 528)
+    assert np.abs(S32 - S64).max() < TOL_MIXED
+    for S in (S64, S32):
+        assert np.linalg.eigvalsh(S.conj().T @ S).max() <= 1 + 1e-9
+    print(f"cfg2: refine iterations {info['refine_iters']}, residual {info['refine_residual']:.2e}", flush=True)
