@@ -12,7 +12,9 @@ reported.  value = 128 * steps / elapsed (whole solves, inputs resident).
 
 roofline: the dominant kernel is the rank-b trailing update of the slice
 inverse (MFMA-bound); its flops per launch are 8 (N-2b)^2 b (6 with 3M), its
-duration is summed from HIP events recorded around each launch on its stream.
+duration is summed from HIP events recorded around one launch in four on its
+stream, inside the timed steps (every launch has the same shape; timing all
+of them would add ~10 us of event latency per block step).
 traffic (HBM bytes per launch) and MFMA utilisation of that kernel come from
 the rocprofv3 counter summary profiles/pmc_slice_inverse_n4096.md (FETCH_SIZE
 and WRITE_SIZE scaled by a copy kernel of known size, MfmaUtil).
@@ -107,7 +109,7 @@ def roofline(info: dict, prec: str):
             "mfma_util_pct": pmc["mfma_util_pct"] if pmc else None,
             "counters_from": os.path.relpath(PMC_SUMMARY, ROOT) if pmc else None,
             "kernel": f"cgemm trailing update of the slice inverse ({prec})",
-            "launches": int(info["kernel_launches"]),
+            "launches_timed": int(info["kernel_launches"]),
             "avg_us": round(1e6 * t / info["kernel_launches"], 2),
             "flops_per_launch": info["kernel_flops"] / info["kernel_launches"]}
 

@@ -94,7 +94,8 @@ typedef struct {
     int32_t block_size;       /* slice-inverse panel width, 0 = tuned default           */
     int32_t complex_3m;       /* FP32 GEMMs: 1 = 3-multiplication complex products     */
     int32_t profile;          /* 1: time the slice-inverse trailing-update kernel with
-                                 HIP events on its stream (fills the kernel_* fields) */
+                                 HIP events on its stream, one launch in four (fills
+                                 the kernel_* fields)                                  */
     int32_t single_stream;    /* 1: no look-ahead second stream in the slice inverse
                                  (same kernels, one stream; for counter collection)   */
     int32_t batch_concurrency;/* rgf_solve_batched: solves in flight at once, each on
@@ -113,7 +114,8 @@ typedef struct {
     double seconds_total, seconds_setup, seconds_slices, seconds_refine;
     double flops_slices;      /* algorithmic flops executed in the slice loop          */
     /* with opt.profile: the dominant kernel (rank-b trailing update of the
-       slice inverse): summed event-timed duration, launches, executed flops */
+       slice inverse), over the launches timed (one in four): summed
+       event-timed duration, number of timed launches, their executed flops */
     double kernel_seconds;
     double kernel_flops;
     int64_t kernel_launches;

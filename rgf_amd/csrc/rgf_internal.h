@@ -51,11 +51,17 @@ struct DevCSR {
 };
 
 // Event timing of one kernel class on the stream it is launched on.
+// Times one launch in every `stride` with a pair of HIP events on its
+// stream (events add latency to the stream); flops and launches count the
+// timed launches only, so flops / seconds estimates the kernel's rate.
 struct KernelTimer {
     std::vector<hipEvent_t> ev;
     size_t used = 0;
     double flops = 0;
-    int64_t launches = 0;
+    int64_t launches = 0;  // timed launches
+    int64_t seen = 0;      // all launches
+    int stride = 4;
+    bool active = false;
     void begin(hipStream_t s);
     void end(hipStream_t s, double kernel_flops);
     double seconds();  // synchronizes on the recorded events

@@ -1431,6 +1431,8 @@ static constexpr int diag_bmax() {
 }
 
 void KernelTimer::begin(hipStream_t s) {
+    active = (seen++ % std::max(stride, 1)) == 0;
+    if (!active) return;
     if (used + 2 > ev.size()) {
         for (int q = 0; q < 64; ++q) {
             hipEvent_t e;
@@ -1442,6 +1444,8 @@ void KernelTimer::begin(hipStream_t s) {
 }
 
 void KernelTimer::end(hipStream_t s, double kernel_flops) {
+    if (!active) return;
+    active = false;
     RGF_HIP(hipEventRecord(ev[used + 1], s));
     used += 2;
     flops += kernel_flops;
