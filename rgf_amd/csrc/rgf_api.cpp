@@ -856,7 +856,9 @@ rgf_status rgf_create(rgf_handle* h, int32_t device) {
         RGF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
         RGF_HIP(hipStreamCreateWithFlags(&c->ov.s2, hipStreamNonBlocking));
-        for (auto& e : c->ov.ev) RGF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        // the events order kernels of one device (a device-scope release is
+        // enough); they may complete with a kernel launch (hipExtLaunchKernel)
+        for (auto& e : c->ov.ev) RGF_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
         RGF_HIP(hipMalloc(&c->d_status, 256));
         RGF_HIP(hipMalloc(&c->d_scratch, 256));
         *h = c.release();

@@ -17,6 +17,8 @@
 
 #include "rgf_internal.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -27,9 +29,10 @@ static int grid_for(int64_t total, int threads = 256, int cap = 16384) {
     return (int)std::max<int64_t>(1, std::min<int64_t>((total + threads - 1) / threads, cap));
 }
 
-// Rows [rlo, rhi) and columns [clo, chi) of C that a GEMM leaves untouched.
+// Rows [rlo, rhi), columns [clo, chi) and the diagonal block
+// [blo, bhi) x [blo, bhi) of C that a GEMM leaves untouched.
 struct Skip {
-    int64_t rlo = 0, rhi = 0, clo = 0, chi = 0;
+    int64_t rlo = 0, rhi = 0, clo = 0, chi = 0, blo = 0, bhi = 0;
 };
 
 // ------------------------------------------------------------------ MFMA wrappers
@@ -96,10 +99,20 @@ struct GemmArgs {
     T* Cre;
     T* Cim;
     int64_t ldc;
+    const T* Xre;  // beta = 1 source of C (C itself unless set otherwise)
+    const T* Xim;
+    int64_t ldx;
+    // optional second destination: C[i][j] for i in [y_rlo, y_rhi), j in
+    // [y_clo, y_chi) is also stored to Y[i - y_rlo][j - y_clo]
+    T* Yre;
+    T* Yim;
+    int64_t ldy;
+    int y_rlo, y_rhi, y_clo, y_chi;
     int M, N, K;
     double alpha_re, alpha_im;
     int beta;
     int srlo, srhi, sclo, schi;
+    int sblo, sbhi;  // skipped diagonal block
     int vec_ok;
     int nz, kc;       // split-K: nz chunks of kc along K, chunk z writes C + z * zstride
     int64_t zstride;
@@ -134,6 +147,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
     const double alpha_re = g.alpha_re, alpha_im = g.alpha_im;
     const int beta = g.beta;
     const int srlo = g.srlo, srhi = g.srhi, sclo = g.sclo, schi = g.schi;
+    const int sblo = g.sblo, sbhi = g.sbhi;
     const int vec_ok = g.vec_ok;
     using V = typename Mfma<T>::v4;
     using VT = typename Vec<T>::type;
@@ -153,7 +167,8 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
     {
         const bool rs = tile_m >= srlo && tile_m + BM <= srhi;
         const bool cs = tile_n >= sclo && tile_n + BN <= schi;
-        if (rs || cs) return;
+        const bool bs = tile_m >= sblo && tile_m + BM <= sbhi && tile_n >= sblo && tile_n + BN <= sbhi;
+        if (rs || cs || bs) return;
     }
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -179,8 +194,8 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
                     const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(l4, r);
                     const int gj = tile_n + wn * TN + b * 16 + l16;
                     if (gi < M && gj < N) {
-                        const int64_t o = (int64_t)gi * ldc + gj;
-                        const T cr = Cre[o], ci = Cim[o];
+                        const int64_t o = (int64_t)gi * g.ldx + gj;
+                        const T cr = g.Xre[o], ci = g.Xim[o];
                         acc0[a][b][r] = cr;
                         if (USE3M)
                             acc2[a][b][r] = cr + ci;  // Im = acc2 - acc0 - acc1 keeps C_im
@@ -373,7 +388,8 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
             for (int r = 0; r < 4; ++r) {
                 const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(l4, r);
                 const int gj = tile_n + wn * TN + b * 16 + l16;
-                const bool masked = (gi >= srlo && gi < srhi) || (gj >= sclo && gj < schi);
+                const bool masked = (gi >= srlo && gi < srhi) || (gj >= sclo && gj < schi) ||
+                                    (gi >= sblo && gi < sbhi && gj >= sblo && gj < sbhi);
                 if (gi < M && gj < N && !masked) {
                     T pr, pi;
                     if (USE3M) {
@@ -384,18 +400,23 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
                         pi = acc1[a][b][r];
                     }
                     const int64_t o = (int64_t)gi * ldc + gj;
-                    if (accc) {
-                        Cre[o] = pr;
-                        Cim[o] = pi;
-                    } else {
+                    if (!accc) {
                         double vr = alpha_re * (double)pr - alpha_im * (double)pi;
                         double vi = alpha_re * (double)pi + alpha_im * (double)pr;
                         if (beta) {
-                            vr += (double)Cre[o];
-                            vi += (double)Cim[o];
+                            const int64_t ox = (int64_t)gi * g.ldx + gj;
+                            vr += (double)g.Xre[ox];
+                            vi += (double)g.Xim[ox];
                         }
-                        Cre[o] = (T)vr;
-                        Cim[o] = (T)vi;
+                        pr = (T)vr;
+                        pi = (T)vi;
+                    }
+                    Cre[o] = pr;
+                    Cim[o] = pi;
+                    if (gi >= g.y_rlo && gi < g.y_rhi && gj >= g.y_clo && gj < g.y_chi) {
+                        const int64_t oy = (int64_t)(gi - g.y_rlo) * g.ldy + (gj - g.y_clo);
+                        g.Yre[oy] = pr;
+                        g.Yim[oy] = pi;
                     }
                 }
             }
@@ -422,15 +443,16 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
 //    order with round-robin XCDs (measured equal or faster: the operand
 //    panels of the shapes used here stay in L2 either way).
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP>
-__global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, int total) {
+__global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, GemmArgs<T> g2,
+                                                              int total) {
     int bid = (int)blockIdx.x;
     if constexpr ((MAP & 1) != 0) {
         const int q = total >> 3, rem = total & 7, x = bid & 7, loc = bid >> 3;
         bid = x * q + min(x, rem) + loc;
     }
-    const bool second = bid >= g0.tiles;
-    const GemmArgs<T>& g = second ? g1 : g0;
-    int lin = second ? bid - g0.tiles : bid;
+    const int which = bid < g0.tiles ? 0 : (bid < g0.tiles + g1.tiles ? 1 : 2);
+    const GemmArgs<T>& g = which == 0 ? g0 : (which == 1 ? g1 : g2);
+    int lin = bid - (which > 0 ? g0.tiles : 0) - (which > 1 ? g1.tiles : 0);
     int z = 0;
     if (g.nz > 1) {
         const int per = g.tiles / g.nz;
@@ -461,6 +483,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     }
     if (tile_m >= g.srlo && tile_m + BM <= g.srhi) return;
     if (tile_n >= g.sclo && tile_n + BN <= g.schi) return;
+    if (tile_m >= g.sblo && tile_m + BM <= g.sbhi && tile_n >= g.sblo && tile_n + BN <= g.sbhi) return;
 
     using V = typename Mfma<T>::v4;
     using VT = typename Vec<T>::type;
@@ -496,8 +519,8 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                 for (int r = 0; r < 4; ++r) {
                     const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(g4, r);
                     if (gi < M && gj < N) {
-                        rr[a][b][r] = Cre[(int64_t)gi * ldc + gj];
-                        ri[a][b][r] = Cim[(int64_t)gi * ldc + gj];
+                        rr[a][b][r] = g.Xre[(int64_t)gi * g.ldx + gj];
+                        ri[a][b][r] = g.Xim[(int64_t)gi * g.ldx + gj];
                     }
                 }
             }
@@ -623,14 +646,20 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
         for (int b = 0; b < FN; ++b) {
             const int gj = tile_n + wn * TN + b * 16 + l16;
             const bool cok = gj < N && !(gj >= g.sclo && gj < g.schi);
+            const bool cblk = gj >= g.sblo && gj < g.sbhi;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(g4, r);
-                if (cok && gi < M && !(gi >= g.srlo && gi < g.srhi)) {
+                if (cok && gi < M && !(gi >= g.srlo && gi < g.srhi) && !(cblk && gi >= g.sblo && gi < g.sbhi)) {
                     T vr = rr[a][b][r];
                     if constexpr (ACC3) vr -= ii[a][b][r];
                     Cre[(int64_t)gi * ldc + gj] = vr;
                     Cim[(int64_t)gi * ldc + gj] = ri[a][b][r];
+                    if (gi >= g.y_rlo && gi < g.y_rhi && gj >= g.y_clo && gj < g.y_chi) {
+                        const int64_t oy = (int64_t)(gi - g.y_rlo) * g.ldy + (gj - g.y_clo);
+                        g.Yre[oy] = vr;
+                        g.Yim[oy] = ri[a][b][r];
+                    }
                 }
             }
         }
@@ -664,6 +693,13 @@ static GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T>
     g.Cre = C.re;
     g.Cim = C.im;
     g.ldc = C.ld;
+    g.Xre = C.re;
+    g.Xim = C.im;
+    g.ldx = C.ld;
+    g.Yre = nullptr;
+    g.Yim = nullptr;
+    g.ldy = 0;
+    g.y_rlo = g.y_rhi = g.y_clo = g.y_chi = 0;
     g.M = (int)M;
     g.N = (int)N;
     g.K = (int)K;
@@ -674,6 +710,8 @@ static GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T>
     g.srhi = (int)sk.rhi;
     g.sclo = (int)sk.clo;
     g.schi = (int)sk.chi;
+    g.sblo = (int)sk.blo;
+    g.sbhi = (int)sk.bhi;
     g.nz = 1;
     g.kc = (int)K;
     g.zstride = 0;
@@ -705,7 +743,8 @@ static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>
 // cgemm2_kernel launch for one or two problems.  Returns false (launching
 // nothing) when a problem is outside the kernel's preconditions.
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4, int MAP = 0>
-static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_) {
+static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_,
+                         const GemmArgs<T>* g2_ = nullptr, hipEvent_t stop = nullptr) {
     constexpr int VW = Vec<T>::W;
     auto fits = [](const GemmArgs<T>& g) {
         const bool unit = g.alpha_im == 0.0 && (g.alpha_re == 1.0 || g.alpha_re == -1.0);
@@ -715,18 +754,30 @@ static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T
         const int64_t nv = ((int64_t)g.N + VW - 1) / VW * VW;
         return unit && kok && g.vec_ok && nv <= g.ldb && g.M > 0;
     };
-    if (!fits(g0_) || (g1_ && !fits(*g1_))) return false;
-    GemmArgs<T> g0 = g0_, g1 = g1_ ? *g1_ : g0_;
+    if (!fits(g0_) || (g1_ && !fits(*g1_)) || (g2_ && !fits(*g2_))) return false;
+    GemmArgs<T> g0 = g0_, g1 = g1_ ? *g1_ : g0_, g2 = g2_ ? *g2_ : g0_;
     auto tiles = [](GemmArgs<T>& g) {
         g.tiles_n = (g.N + BN - 1) / BN;
         g.tiles = g.tiles_n * ((g.M + BM - 1) / BM) * g.nz;
     };
     tiles(g0);
     tiles(g1);
-    const int64_t nblocks = (int64_t)g0.tiles + (g1_ ? g1.tiles : 0);
-    if (nblocks <= 0) return true;
-    hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP>), dim3((unsigned)nblocks), dim3(WM * WN * 64),
-                       0, s, g0, g1, (int)nblocks);
+    tiles(g2);
+    if (!g1_) g1.tiles = 0;
+    if (!g2_) g2.tiles = 0;
+    const int64_t nblocks = (int64_t)g0.tiles + g1.tiles + g2.tiles;
+    if (nblocks <= 0) {
+        if (stop) RGF_HIP(hipEventRecord(stop, s));
+        return true;
+    }
+    if (stop) {
+        // the event completes with the kernel (no separate marker packet)
+        hipExtLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP>), dim3((unsigned)nblocks),
+                              dim3(WM * WN * 64), 0, s, nullptr, stop, 0, g0, g1, g2, (int)nblocks);
+    } else {
+        hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP>), dim3((unsigned)nblocks),
+                           dim3(WM * WN * 64), 0, s, g0, g1, g2, (int)nblocks);
+    }
     return true;
 }
 
@@ -765,7 +816,7 @@ template <typename T>
 void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
                 int beta, Skip sk, int use3m, T* ws, size_t ws_elems) {
     if (M <= 0 || N <= 0) return;
-    const bool any_skip = sk.rhi > sk.rlo || sk.chi > sk.clo;
+    const bool any_skip = sk.rhi > sk.rlo || sk.chi > sk.clo || sk.bhi > sk.blo;
     // split-K for thin products with a long inner dimension
     if (ws && !any_skip && K >= 1024) {
         const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
@@ -820,12 +871,37 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
     RGF_HIP(hipGetLastError());
 }
 
-// Two small products in one launch.
+// Two small products in one launch; `stop`, if given, is recorded when the
+// launch completes.
 template <typename T>
-static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, int use3m) {
-    if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g0, &g1))
+static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, int use3m,
+                       hipEvent_t stop = nullptr) {
+    if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g0, &g1, nullptr, stop)) {
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
+        if (stop) RGF_HIP(hipEventRecord(stop, s));
+    }
     RGF_HIP(hipGetLastError());
+}
+
+// Three independent products in one launch (two launches on the fallback).
+template <typename T>
+static void cgemm_triple(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, const GemmArgs<T>& g2,
+                         int use3m, hipEvent_t stop = nullptr) {
+    if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g0, &g1, &g2, stop)) {
+        launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
+        launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g2, nullptr, use3m);
+        if (stop) RGF_HIP(hipEventRecord(stop, s));
+    }
+    RGF_HIP(hipGetLastError());
+}
+
+// gemm_args with the beta = 1 source X in place of C
+template <typename T>
+static GemmArgs<T> from_source(GemmArgs<T> g, CMat<T> X) {
+    g.Xre = X.re;
+    g.Xim = X.im;
+    g.ldx = X.ld;
+    return g;
 }
 
 template <typename T>
@@ -1425,6 +1501,26 @@ __global__ void gj_writeback2_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int
     }
 }
 
+// Copies of the block column A[:, p0:p0+nb] (to C, ld ldc) and block row
+// A[p0:p0+nb, :] (to R, ld ldr), both planes.
+template <typename T>
+__global__ void panel_copy_kernel(const T* Are, const T* Aim, int64_t lda, int64_t N, int64_t p0, int nb, T* Cre,
+                                  T* Cim, int64_t ldc, T* Rre, T* Rim, int64_t ldr) {
+    const int64_t half = (int64_t)nb * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < 2 * half;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < half) {
+            const int64_t i = e / nb, j = e % nb;
+            Cre[i * ldc + j] = Are[i * lda + p0 + j];
+            Cim[i * ldc + j] = Aim[i * lda + p0 + j];
+        } else {
+            const int64_t f = e - half, i = f / N, j = f % N;
+            Rre[i * ldr + j] = Are[(p0 + i) * lda + j];
+            Rim[i * ldr + j] = Aim[(p0 + i) * lda + j];
+        }
+    }
+}
+
 template <typename T>
 static constexpr int diag_bmax() {
     return sizeof(T) == 8 ? 64 : 128;  // the diagonal block lives in LDS
@@ -1436,7 +1532,7 @@ void KernelTimer::begin(hipStream_t s) {
     if (used + 2 > ev.size()) {
         for (int q = 0; q < 64; ++q) {
             hipEvent_t e;
-            RGF_HIP(hipEventCreate(&e));
+            RGF_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
             ev.push_back(e);
         }
     }
@@ -1490,9 +1586,9 @@ static void launch_diag(hipStream_t s, CMat<T> A, int64_t p0, int nb, CMat<T> D,
 //   D = A_PP^-1,  R = D A_P:,  C = -A_:P D,  A_JJ -= A_JP R_:J,  A_PJ = R_PJ,
 //   A_JP = C_JP,  A_PP = D.
 // With a second stream (look-ahead), the bulk update of step p runs there
-// while the first stream updates block row/column p+1, writes step p's panels
-// back and factors block p+1, so the single-work-group diagonal inverse and
-// the panel products overlap the bulk update.
+// while the first stream forms the panels of block p+1 and factors block
+// p+2, so the single-work-group diagonal inverse and the panel products
+// overlap the bulk update.
 template <typename T>
 void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_status, int use3m,
                     KernelTimer* timer, const Overlap* ov) {
@@ -1544,50 +1640,126 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         RGF_HIP(hipGetLastError());
         return;
     }
-    // Look-ahead.  Per block q (P = block q, Q = block q+1):
-    //  first stream : after the bulk update of q-1, update block row and
-    //                 column Q with step q (one launch, runs alone), then
-    //                 invert block Q at once, so that its single work group is
-    //                 dispatched before the bulk update fills the CUs;
-    //  second stream: bulk update of everything outside P and Q, reading the
-    //                 old column panel in place, then write the column panel
-    //                 back outside P and Q;
-    //  first stream : meanwhile write the row panel and the Q rows of the
-    //                 column panel back and form Q's panels.
-    // The diagonal block kernel writes only D (its own buffer) and A_QQ.
+    // Look-ahead on two streams.  Per block q (P = block q, Q = q+1, Q2 = q+2):
+    //  second stream: one launch brings block row and column Q up to date
+    //                 with step q (outside P and Q, written to the panel
+    //                 copies Rold and Aold[Q]) and applies step q to the
+    //                 block (Q2, Q2) in place; then the bulk update of step
+    //                 q outside P, Q and (Q2, Q2);
+    //  first stream : forms Q's panels in place from the copies (the blocks
+    //                 A[Q, P] and A[P, Q] reached the copies with step q's
+    //                 panels), applies step q+1 to (Q2, Q2) and queues its
+    //                 inverse, which is dispatched beside the bulk update of
+    //                 step q (its single work group needs a whole CU).
+    // Every write to A is in place and no write-back follows the bulk update;
+    // the second stream waits only for the panels of the step it applies.
     hipStream_t s2 = ov->s2;
-    panels(s, 0, (int)std::min<int64_t>(b, N), 0);
-    for (int64_t q = 0; q < nblk; ++q) {
-        const int64_t p0 = q * b;
-        const int nb = (int)std::min<int64_t>(b, N - p0);
-        const int st = (int)(q & 1);
-        const bool next = q + 1 < nblk;
-        const int64_t p1 = p0 + nb;
-        const int nb1 = next ? (int)std::min<int64_t>(b, N - p1) : 0;
-        if (q > 0) RGF_HIP(hipStreamWaitEvent(s, ov->ev[2 + (st ^ 1)], 0));
-        if (next) {
-            const GemmArgs<T> ga = gemm_args<T>(nb1, N, nb, A.offset(p1, p0), R[st], A.offset(p1, 0), cd(-1, 0), 1,
-                                                Skip{0, 0, p0, p1});
-            const GemmArgs<T> gb = gemm_args<T>(N, nb1, nb, A.offset(0, p0), R[st].offset(0, p1), A.offset(0, p1),
-                                                cd(-1, 0), 1, Skip{p0, p1 + nb1, 0, 0});
-            cgemm_pair<T>(s, ga, gb, use3m);
+    hipEvent_t evS[2] = {ov->ev[0], ov->ev[1]}, evT[2] = {ov->ev[2], ov->ev[3]};
+    CMat<T> Aold[2], Rold;
+    {
+        // workspace: Aold[0], Aold[1] (N x b), Rold (b x N), then D
+        const size_t pl = (size_t)3 * b * N + (size_t)b * b;
+        Aold[0] = CMat<T>(ws, ws + pl, b);
+        Aold[1] = CMat<T>(ws + (size_t)b * N, ws + pl + (size_t)b * N, b);
+        Rold = CMat<T>(ws + (size_t)2 * b * N, ws + pl + (size_t)2 * b * N, N);
+        D = CMat<T>(ws + (size_t)3 * b * N, ws + pl + (size_t)3 * b * N, b);
+    }
+    auto p_of = [&](int64_t q) { return q * b; };
+    auto nb_of = [&](int64_t q) { return q < nblk ? (int)std::min<int64_t>(b, N - q * b) : 0; };
+    // A[P, J] = D Rold[:, J] and A[I, P] = -Aold[I, :] D outside P.  The
+    // blocks the next copies need from these panels, A[Q, P] and A[P, Q], are
+    // also stored to Rold[:, P] and Aold[Q][P, :] (Rold[:, P] is not read
+    // here: the tiles of columns P are skipped).
+    auto panels_inplace = [&](int64_t q, hipEvent_t stop) {
+        const int64_t p0 = p_of(q), p1 = p0 + nb_of(q);
+        const int nb = nb_of(q), nb1 = nb_of(q + 1);
+        GemmArgs<T> gr = gemm_args<T>(nb, N, nb, D, Rold, A.offset(p0, 0), cd(1, 0), 0, Skip{0, 0, p0, p1});
+        GemmArgs<T> gc = gemm_args<T>(N, nb, nb, Aold[q & 1], D, A.offset(0, p0), cd(-1, 0), 0, Skip{p0, p1, 0, 0});
+        if (nb1 > 0) {
+            const CMat<T> An = Aold[(q + 1) & 1].offset(p0, 0), Rp = Rold.offset(0, p0);
+            gr.Yre = An.re;
+            gr.Yim = An.im;
+            gr.ldy = An.ld;
+            gr.y_rlo = 0;
+            gr.y_rhi = nb;
+            gr.y_clo = (int)p1;
+            gr.y_chi = (int)(p1 + nb1);
+            gc.Yre = Rp.re;
+            gc.Yim = Rp.im;
+            gc.ldy = Rp.ld;
+            gc.y_rlo = (int)p1;
+            gc.y_rhi = (int)(p1 + nb1);
+            gc.y_clo = 0;
+            gc.y_chi = nb;
         }
-        RGF_HIP(hipEventRecord(ov->ev[st], s));
-        if (next) launch_diag<T>(s, A, p1, nb1, D, dev_status);
-        RGF_HIP(hipStreamWaitEvent(s2, ov->ev[st], 0));
+        cgemm_pair<T>(s, gr, gc, use3m, stop);
+    };
+    // A[K, K] -= Aold_j[K, :] A[Pj, K]: step j applied to diagonal block k;
+    // `stop` is recorded when it completes
+    auto schur = [&](int64_t k, int64_t j, hipEvent_t stop) {
+        const int64_t pk = p_of(k), pj = p_of(j);
+        const int nk = nb_of(k), nj = nb_of(j);
+        const GemmArgs<T> g = gemm_args<T>(nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk),
+                                           cd(-1, 0), 1, Skip{});
+        if (use3m || !launch_pair2<T, 32, 32, 16, 1, 1, false>(s, g, nullptr, nullptr, stop)) {
+            cgemm_skip<T>(s, nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk), cd(-1, 0), 1,
+                          Skip{}, use3m, nullptr, 0);
+            if (stop) RGF_HIP(hipEventRecord(stop, s));
+        }
+    };
+    {
+        // step 0 panels from full copies of block row and column 0
+        const int nb = nb_of(0);
+        const int64_t tot = 2 * (int64_t)nb * N;
+        launch_diag<T>(s, A, 0, nb, D, dev_status);
+        hipLaunchKernelGGL((panel_copy_kernel<T>), dim3(grid_for(tot, 256, 4096)), dim3(256), 0, s, A.re, A.im, A.ld,
+                           N, (int64_t)0, nb, Aold[0].re, Aold[0].im, Aold[0].ld, Rold.re, Rold.im, Rold.ld);
+        panels_inplace(0, nullptr);
+        schur(1, 0, evS[0]);
+        launch_diag<T>(s, A, p_of(1), nb_of(1), D, dev_status);
+    }
+    for (int64_t q = 0; q < nblk; ++q) {
+        const int64_t p0 = p_of(q), p1 = p_of(q + 1);
+        const int nb = nb_of(q), nb1 = nb_of(q + 1), nb2 = nb_of(q + 2);
+        const int64_t p2 = p1 + nb1, p3 = p2 + nb2;
+        CMat<T> An = Aold[(q + 1) & 1];
+        RGF_HIP(hipStreamWaitEvent(s2, evS[q & 1], 0));
+        if (nb1 > 0) {
+            // Rold = A[Q, J] - Aold[Q, :] A[P, J]; An[I, :] = A[I, Q] - Aold[I, :] A[P, Q]
+            const GemmArgs<T> ga = from_source(gemm_args<T>(nb1, N, nb, Aold[q & 1].offset(p1, 0), A.offset(p0, 0),
+                                                            Rold, cd(-1, 0), 1, Skip{0, 0, p0, p2}),
+                                               A.offset(p1, 0));
+            const GemmArgs<T> gb = from_source(gemm_args<T>(N, nb1, nb, Aold[q & 1], A.offset(p0, p1), An, cd(-1, 0), 1,
+                                                            Skip{p0, p2, 0, 0}),
+                                               A.offset(0, p1));
+            if (nb2 > 0) {
+                const GemmArgs<T> gc = gemm_args<T>(nb2, nb2, nb, Aold[q & 1].offset(p2, 0), A.offset(p0, p2),
+                                                    A.offset(p2, p2), cd(-1, 0), 1, Skip{});
+                cgemm_triple<T>(s2, ga, gb, gc, use3m, evT[q & 1]);
+            } else {
+                cgemm_pair<T>(s2, ga, gb, use3m, evT[q & 1]);
+            }
+        }
         if (timer) timer->begin(s2);
-        cgemm_skip<T>(s2, N, N, nb, A.offset(0, p0), R[st], A, cd(-1, 0), 1, Skip{p0, p1 + nb1, p0, p1 + nb1}, use3m,
-                      nullptr, 0);
+        cgemm_skip<T>(s2, N, N, nb, Aold[q & 1], A.offset(p0, 0), A, cd(-1, 0), 1, Skip{p0, p2, p0, p2, p2, p3},
+                      use3m, nullptr, 0);
         if (timer) {
             const double nn = (double)(N - nb - nb1);
-            timer->end(s2, fl * nn * nn * nb);
+            timer->end(s2, fl * (nn * nn - (double)nb2 * nb2) * nb);
         }
-        writeback(s2, p0, nb, st, 0, 0, N, p0, p1 + nb1);
-        RGF_HIP(hipEventRecord(ov->ev[2 + st], s2));
-        writeback(s, p0, nb, st, 1, p1, p1 + nb1, 0, 0);
-        if (next) panel_products(s, p1, nb1, st ^ 1);
+        if (nb1 > 0) {
+            RGF_HIP(hipStreamWaitEvent(s, evT[q & 1], 0));
+            if (nb2 > 0) {
+                panels_inplace(q + 1, nullptr);
+                schur(q + 2, q + 1, evS[(q + 1) & 1]);
+                launch_diag<T>(s, A, p2, nb2, D, dev_status);
+            } else {
+                panels_inplace(q + 1, evS[(q + 1) & 1]);
+            }
+        }
     }
-    RGF_HIP(hipStreamWaitEvent(s, ov->ev[2 + (int)((nblk - 1) & 1)], 0));
+    RGF_HIP(hipEventRecord(ov->ev[4], s2));
+    RGF_HIP(hipStreamWaitEvent(s, ov->ev[4], 0));
     RGF_HIP(hipGetLastError());
 }
 

@@ -41,7 +41,8 @@ def _well_posed(rng, n):
     return _crand(rng, n, n) / np.sqrt(n) + 4 * np.eye(n)
 
 
-@pytest.mark.parametrize("n,b", [(1, 0), (5, 0), (64, 0), (100, 16), (200, 32), (333, 64), (700, 0)])
+@pytest.mark.parametrize("n,b", [(1, 0), (5, 0), (64, 0), (100, 16), (200, 32), (333, 64), (700, 0), (129, 64),
+                                 (130, 64), (192, 64)])
 def test_slice_inverse_fp64(n, b):
     rng = np.random.default_rng(n + b)
     A = _well_posed(rng, n)
@@ -49,7 +50,8 @@ def test_slice_inverse_fp64(n, b):
     assert np.abs(X @ A - np.eye(n)).max() < 1e-12
 
 
-@pytest.mark.parametrize("n,b", [(7, 0), (128, 0), (300, 64), (1000, 128)])
+@pytest.mark.parametrize("n,b", [(7, 0), (128, 0), (300, 64), (1000, 128), (200, 128), (301, 64), (520, 128),
+                                 (384, 128)])
 def test_slice_inverse_fp32(n, b):
     rng = np.random.default_rng(n + b + 1)
     A = _well_posed(rng, n)
