@@ -442,7 +442,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
 //    run column-major in groups of 8 tile rows; MAP 0 keeps the row-major
 //    order with round-robin XCDs (measured equal or faster: the operand
 //    panels of the shapes used here stay in L2 either way).
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR>
 __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, GemmArgs<T> g2,
                                                               int total) {
     int bid = (int)blockIdx.x;
@@ -509,21 +509,60 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
             if constexpr (ACC3) ii[a][b] = V{0, 0, 0, 0};
         }
 
+    // TR: the MFMA computes tiles of C^T (B as its first operand), so the
+    // four accumulator elements of a lane are four consecutive columns of
+    // one row of C, C[l16][4 g4 + r]: 16-byte loads and stores of C.  The B
+    // operand column feeding MFMA row i is bperm(i) (FP64 accumulators hold
+    // rows g4 + 4r, which the permutation sends to columns 4 g4 + r).
+    auto bperm = [](int i) { return (TR && sizeof(T) == 8) ? 4 * (i & 3) + (i >> 2) : i; };
     if (g.beta) {
+        if constexpr (TR) {
+            const bool vx = (g.ldx % 4) == 0 && (((uintptr_t)g.Xre | (uintptr_t)g.Xim) & 15) == 0;
 #pragma unroll
-        for (int a = 0; a < FM; ++a)
+            for (int a = 0; a < FM; ++a)
 #pragma unroll
-            for (int b = 0; b < FN; ++b) {
-                const int gj = tile_n + wn * TN + b * 16 + l16;
+                for (int b = 0; b < FN; ++b) {
+                    const int gi = tile_m + wm * TM + a * 16 + l16;
+                    const int gj = tile_n + wn * TN + b * 16 + 4 * g4;
+                    if (gi < M) {
+                        const int64_t o = (int64_t)gi * g.ldx + gj;
+                        if (vx && gj + 4 <= N) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(g4, r);
-                    if (gi < M && gj < N) {
-                        rr[a][b][r] = g.Xre[(int64_t)gi * g.ldx + gj];
-                        ri[a][b][r] = g.Xim[(int64_t)gi * g.ldx + gj];
+                            for (int u = 0; u < 4; u += VW) {
+                                const VT xr = *(const VT*)(g.Xre + o + u);
+                                const VT xi = *(const VT*)(g.Xim + o + u);
+#pragma unroll
+                                for (int v = 0; v < VW; ++v) {
+                                    rr[a][b][u + v] = xr[v];
+                                    ri[a][b][u + v] = xi[v];
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                if (gj + r < N) {
+                                    rr[a][b][r] = g.Xre[o + r];
+                                    ri[a][b][r] = g.Xim[o + r];
+                                }
+                        }
                     }
                 }
-            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < FM; ++a)
+#pragma unroll
+                for (int b = 0; b < FN; ++b) {
+                    const int gj = tile_n + wn * TN + b * 16 + l16;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int gi = tile_m + wm * TM + a * 16 + Mfma<T>::row(g4, r);
+                        if (gi < M && gj < N) {
+                            rr[a][b][r] = g.Xre[(int64_t)gi * g.ldx + gj];
+                            ri[a][b][r] = g.Xim[(int64_t)gi * g.ldx + gj];
+                        }
+                    }
+                }
+        }
     }
 
     // global -> register staging of one K chunk; B column vectors are read up
@@ -601,38 +640,40 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
             }
 #pragma unroll
             for (int b = 0; b < FN; ++b) {
-                const int col = wn * TN + b * 16 + l16;
+                const int col = wn * TN + b * 16 + bperm(l16);
 #pragma unroll
                 for (int s2 = 0; s2 < 4; ++s2) {
                     br[b][s2] = sB[buf][0][kb + 4 * g4 + s2][col];
                     bi[b][s2] = sB[buf][1][kb + 4 * g4 + s2][col];
                 }
             }
+            // x * y accumulated into the tile of C (TR: of C^T)
+            auto mm = [](T x, T y, V c) { return TR ? Mfma<T>::mma(y, x, c) : Mfma<T>::mma(x, y, c); };
 #pragma unroll
             for (int s2 = 0; s2 < 4; ++s2) {
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
-                    for (int b = 0; b < FN; ++b) rr[a][b] = Mfma<T>::mma(ar[a][s2], br[b][s2], rr[a][b]);
+                    for (int b = 0; b < FN; ++b) rr[a][b] = mm(ar[a][s2], br[b][s2], rr[a][b]);
                 if constexpr (ACC3) {
 #pragma unroll
                     for (int a = 0; a < FM; ++a)
 #pragma unroll
-                        for (int b = 0; b < FN; ++b) ii[a][b] = Mfma<T>::mma(ai[a][s2], bi[b][s2], ii[a][b]);
+                        for (int b = 0; b < FN; ++b) ii[a][b] = mm(ai[a][s2], bi[b][s2], ii[a][b]);
                 } else {
 #pragma unroll
                     for (int a = 0; a < FM; ++a)
 #pragma unroll
-                        for (int b = 0; b < FN; ++b) rr[a][b] = Mfma<T>::mma(-ai[a][s2], bi[b][s2], rr[a][b]);
+                        for (int b = 0; b < FN; ++b) rr[a][b] = mm(-ai[a][s2], bi[b][s2], rr[a][b]);
                 }
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
-                    for (int b = 0; b < FN; ++b) ri[a][b] = Mfma<T>::mma(ar[a][s2], bi[b][s2], ri[a][b]);
+                    for (int b = 0; b < FN; ++b) ri[a][b] = mm(ar[a][s2], bi[b][s2], ri[a][b]);
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
-                    for (int b = 0; b < FN; ++b) ri[a][b] = Mfma<T>::mma(ai[a][s2], br[b][s2], ri[a][b]);
+                    for (int b = 0; b < FN; ++b) ri[a][b] = mm(ai[a][s2], br[b][s2], ri[a][b]);
             }
         }
         if (kc + 1 < nk) stash(buf ^ 1);
@@ -640,6 +681,63 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     }
 
     // epilogue (the accumulators started from C when beta = 1)
+    if constexpr (TR) {
+        const bool vc = (ldc % 4) == 0 && (((uintptr_t)Cre | (uintptr_t)Cim) & 15) == 0;
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+                const int gi = tile_m + wm * TM + a * 16 + l16;
+                const int gj = tile_n + wn * TN + b * 16 + 4 * g4;
+                if (gi >= M || (gi >= g.srlo && gi < g.srhi)) continue;
+                const bool rblk = gi >= g.sblo && gi < g.sbhi;
+                T vr[4], vi[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    vr[r] = rr[a][b][r];
+                    if constexpr (ACC3) vr[r] -= ii[a][b][r];
+                    vi[r] = ri[a][b][r];
+                }
+                const int64_t o = (int64_t)gi * ldc + gj;
+                const bool whole = vc && gj + 4 <= N && !(gj + 4 > g.sclo && gj < g.schi) &&
+                                   !(rblk && gj + 4 > g.sblo && gj < g.sbhi);
+                if (whole) {
+#pragma unroll
+                    for (int u = 0; u < 4; u += VW) {
+                        VT xr, xi;
+#pragma unroll
+                        for (int v = 0; v < VW; ++v) {
+                            xr[v] = vr[u + v];
+                            xi[v] = vi[u + v];
+                        }
+                        *(VT*)(Cre + o + u) = xr;
+                        *(VT*)(Cim + o + u) = xi;
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int c = gj + r;
+                        if (c < N && !(c >= g.sclo && c < g.schi) && !(rblk && c >= g.sblo && c < g.sbhi)) {
+                            Cre[o + r] = vr[r];
+                            Cim[o + r] = vi[r];
+                        }
+                    }
+                }
+                if (gi >= g.y_rlo && gi < g.y_rhi) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int c = gj + r;
+                        if (c < N && c >= g.y_clo && c < g.y_chi && !(c >= g.sclo && c < g.schi) &&
+                            !(rblk && c >= g.sblo && c < g.sbhi)) {
+                            const int64_t oy = (int64_t)(gi - g.y_rlo) * g.ldy + (c - g.y_clo);
+                            g.Yre[oy] = vr[r];
+                            g.Yim[oy] = vi[r];
+                        }
+                    }
+                }
+            }
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -742,7 +840,8 @@ static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>
 
 // cgemm2_kernel launch for one or two problems.  Returns false (launching
 // nothing) when a problem is outside the kernel's preconditions.
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4, int MAP = 0>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4, int MAP = 0,
+          bool TR = true>
 static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_,
                          const GemmArgs<T>* g2_ = nullptr, hipEvent_t stop = nullptr) {
     constexpr int VW = Vec<T>::W;
@@ -772,10 +871,10 @@ static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T
     }
     if (stop) {
         // the event completes with the kernel (no separate marker packet)
-        hipExtLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP>), dim3((unsigned)nblocks),
+        hipExtLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR>), dim3((unsigned)nblocks),
                               dim3(WM * WN * 64), 0, s, nullptr, stop, 0, g0, g1, g2, (int)nblocks);
     } else {
-        hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP>), dim3((unsigned)nblocks),
+        hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR>), dim3((unsigned)nblocks),
                            dim3(WM * WN * 64), 0, s, g0, g1, g2, (int)nblocks);
     }
     return true;
