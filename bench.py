@@ -126,7 +126,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--skip-fp64", action="store_true", help="do not time the plain FP64 path")
     ap.add_argument("--cpu-slices", type=int, default=2)
-    ap.add_argument("--complex-3m", type=int, default=0, help="3-multiplication complex GEMMs in FP32")
+    ap.add_argument("--complex-3m", type=int, default=None,
+                    help="3-multiplication complex GEMMs in FP32: bit 0 slice inverses, bit 1 refinement sweeps "
+                         "(default: the library's, 2)")
     ap.add_argument("--block", type=int, default=0, help="slice-inverse panel width (0 = default)")
     ap.add_argument("--single-stream", action="store_true",
                     help="no look-ahead stream in the slice inverse (for hardware-counter runs)")
@@ -184,6 +186,7 @@ def main():
         "roofline": roofline(agg, "fp32"),
         "effective_tflops_reference_formulation": round(value * 32.0 * float(N) ** 3 / 1e12, 1),
         "refine_tol": args.refine_tol,
+        "complex_3m": int(rgf_amd.api._options("fp32_refined", complex_3m=args.complex_3m).complex_3m),
         "refine_iters": infos[-1]["refine_iters"], "refine_residual": infos[-1]["refine_residual"],
         "unitarity_defect_fp32_refined": unitarity(S32),
     }

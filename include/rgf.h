@@ -92,7 +92,10 @@ typedef struct {
     int32_t max_refine_iter;  /* FP32_REFINED: correction sweeps (default 6)           */
     double refine_tol;        /* FP32_REFINED: stop at max|r|/max|b| below (1e-13)     */
     int32_t block_size;       /* slice-inverse panel width, 0 = tuned default           */
-    int32_t complex_3m;       /* FP32 GEMMs: 1 = 3-multiplication complex products     */
+    int32_t complex_3m;       /* FP32 GEMMs with 3 real products per complex product:
+                                 bit 0 = in the slice inverses, bit 1 = in the
+                                 refinement sweeps (default 2: sweeps only, which
+                                 leaves the convergence rate unchanged)               */
     int32_t profile;          /* 1: time the slice-inverse trailing-update kernel with
                                  HIP events on its stream, one launch in four (fills
                                  the kernel_* fields)                                  */

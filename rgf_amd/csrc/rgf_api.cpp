@@ -389,7 +389,9 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     // solves H psi = 0 and stays zero)
     const int64_t mL = w.inL.size(), mR = w.inR.size(), m = (mL + mR + 3) / 4 * 4;
     const int b = o.block_size > 0 ? o.block_size : default_block(RGF_PRECISION_FP32_REFINED);
-    const int use3m = o.complex_3m;
+    // three real products per complex product: bit 0 in the slice inverses,
+    // bit 1 in the refinement sweeps
+    const int use3m = o.complex_3m & 1, sweep3m = (o.complex_3m >> 1) & 1;
     hipStream_t s = c->stream;
     // factor store: S slices of N x N complex FP32
     size_t freeb = 0, totalb = 0;
@@ -457,16 +459,16 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         for (int64_t n = 0; n < S; ++n) {
             convert<float, double>(s, N, m, P(r, n), t, sc);
             if (n > 0) add_mat<float>(s, N, m, t, D(n - 1), t);
-            cgemm<float>(s, N, m, N, Fn(n), t, D(n), cd(1, 0), 0, 0, 0, use3m, wk.first, wk.second);
+            cgemm<float>(s, N, m, N, Fn(n), t, D(n), cd(1, 0), 0, 0, 0, sweep3m, wk.first, wk.second);
         }
         // backward sweep: d_n += g_n d_{n+1}
         for (int64_t n = S - 2; n >= 0; --n)
-            cgemm<float>(s, N, m, N, Fn(n), D(n + 1), D(n), cd(1, 0), 1, 0, 0, use3m, wk.first, wk.second);
+            cgemm<float>(s, N, m, N, Fn(n), D(n + 1), D(n), cd(1, 0), 1, 0, 0, sweep3m, wk.first, wk.second);
         accumulate<float>(s, S * N, m, CMat<float>(d, d + S * SE, m), CMat<double>(psi, psi + S * SE, m), rmax);
     }
     w.refine_iters = it;
     w.refine_residual = rel;
-    w.flops += (double)it * (2.0 * S - 1) * 8 * Nd * Nd * m * (use3m ? 0.75 : 1.0);
+    w.flops += (double)it * (2.0 * S - 1) * 8 * Nd * Nd * m * (sweep3m ? 0.75 : 1.0);
     // S blocks from the boundary slices: C_L psi_0, C_R psi_{L+1}
     const int64_t po = w.outL.size(), qo = w.outR.size();
     if (po) {
@@ -839,7 +841,7 @@ void rgf_default_options(rgf_options* o) {
     o->max_refine_iter = 6;
     o->refine_tol = 1e-13;
     o->block_size = 0;
-    o->complex_3m = 0;
+    o->complex_3m = 2;  // three real products in the refinement sweeps only
 }
 
 rgf_status rgf_create(rgf_handle* h, int32_t device) {

@@ -442,7 +442,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
 //    run column-major in groups of 8 tile rows; MAP 0 keeps the row-major
 //    order with round-robin XCDs (measured equal or faster: the operand
 //    panels of the shapes used here stay in L2 either way).
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR, bool M3>
 __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, GemmArgs<T> g2,
                                                               int total) {
     int bid = (int)blockIdx.x;
@@ -499,14 +499,18 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     const int l16 = lane & 15, g4 = lane >> 4;
     const T sgn = (T)g.alpha_re;  // +-1
 
-    V rr[FM][FN], ii[ACC3 ? FM : 1][ACC3 ? FN : 1], ri[FM][FN];
+    // M3 (three real products): rr = ArBr, ii = AiBi, ri = (Ar + Ai)(Br + Bi);
+    // Re = rr - ii, Im = ri - rr - ii (C enters as rr = Cr, ri = Cr + Ci)
+    static_assert(!M3 || TR, "3M products use the transposed accumulators");
+    constexpr bool A3 = ACC3 || M3;
+    V rr[FM][FN], ii[A3 ? FM : 1][A3 ? FN : 1], ri[FM][FN];
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
         for (int b = 0; b < FN; ++b) {
             rr[a][b] = V{0, 0, 0, 0};
             ri[a][b] = V{0, 0, 0, 0};
-            if constexpr (ACC3) ii[a][b] = V{0, 0, 0, 0};
+            if constexpr (A3) ii[a][b] = V{0, 0, 0, 0};
         }
 
     // TR: the MFMA computes tiles of C^T (B as its first operand), so the
@@ -545,6 +549,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                                     ri[a][b][r] = g.Xim[o + r];
                                 }
                         }
+                        if constexpr (M3) ri[a][b] += rr[a][b];
                     }
                 }
         } else {
@@ -649,6 +654,28 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
             }
             // x * y accumulated into the tile of C (TR: of C^T)
             auto mm = [](T x, T y, V c) { return TR ? Mfma<T>::mma(y, x, c) : Mfma<T>::mma(x, y, c); };
+            if constexpr (M3) {
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b) rr[a][b] = mm(ar[a][s2], br[b][s2], rr[a][b]);
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b) ii[a][b] = mm(ai[a][s2], bi[b][s2], ii[a][b]);
+                    T sa[FM], sb[FN];
+#pragma unroll
+                    for (int a = 0; a < FM; ++a) sa[a] = ar[a][s2] + ai[a][s2];
+#pragma unroll
+                    for (int b = 0; b < FN; ++b) sb[b] = br[b][s2] + bi[b][s2];
+#pragma unroll
+                    for (int a = 0; a < FM; ++a)
+#pragma unroll
+                        for (int b = 0; b < FN; ++b) ri[a][b] = mm(sa[a], sb[b], ri[a][b]);
+                }
+            } else
 #pragma unroll
             for (int s2 = 0; s2 < 4; ++s2) {
 #pragma unroll
@@ -695,8 +722,9 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     vr[r] = rr[a][b][r];
-                    if constexpr (ACC3) vr[r] -= ii[a][b][r];
+                    if constexpr (A3) vr[r] -= ii[a][b][r];
                     vi[r] = ri[a][b][r];
+                    if constexpr (M3) vi[r] -= rr[a][b][r] + ii[a][b][r];
                 }
                 const int64_t o = (int64_t)gi * ldc + gj;
                 const bool whole = vc && gj + 4 <= N && !(gj + 4 > g.sclo && gj < g.schi) &&
@@ -841,7 +869,7 @@ static void launch_pair(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>
 // cgemm2_kernel launch for one or two problems.  Returns false (launching
 // nothing) when a problem is outside the kernel's preconditions.
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3 = sizeof(T) == 4, int MAP = 0,
-          bool TR = true>
+          bool TR = true, bool M3 = false>
 static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T>* g1_,
                          const GemmArgs<T>* g2_ = nullptr, hipEvent_t stop = nullptr) {
     constexpr int VW = Vec<T>::W;
@@ -871,13 +899,21 @@ static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T
     }
     if (stop) {
         // the event completes with the kernel (no separate marker packet)
-        hipExtLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR>), dim3((unsigned)nblocks),
+        hipExtLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3>), dim3((unsigned)nblocks),
                               dim3(WM * WN * 64), 0, s, nullptr, stop, 0, g0, g1, g2, (int)nblocks);
     } else {
-        hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR>), dim3((unsigned)nblocks),
+        hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3>), dim3((unsigned)nblocks),
                            dim3(WM * WN * 64), 0, s, g0, g1, g2, (int)nblocks);
     }
     return true;
+}
+
+// launch_pair2 with four (use3m = 0) or three (use3m != 0) real products
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static bool launch_cplx(int use3m, hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>* g1 = nullptr,
+                        const GemmArgs<T>* g2 = nullptr, hipEvent_t stop = nullptr) {
+    if (use3m) return launch_pair2<T, BM, BN, BK, WM, WN, false, 0, true, true>(s, g0, g1, g2, stop);
+    return launch_pair2<T, BM, BN, BK, WM, WN, false>(s, g0, g1, g2, stop);
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN>
@@ -932,7 +968,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
                 g.nz = (int)nz;
                 g.kc = (int)kc;
                 g.zstride = per;
-                if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g, nullptr))
+                if (!launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g))
                     launch_pair<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, g, nullptr, use3m);
                 hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
                                    (const T*)ws, per, C.re, C.im, C.ld, alpha.real(), alpha.imag(), beta);
@@ -948,12 +984,12 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
     if (M > INT32_MAX || N > INT32_MAX || K > INT32_MAX) throw Error(RGF_ERR_INVALID_ARGUMENT, "GEMM too large");
     const int64_t tiles_big = ((M + 127) / 128) * ((N + 127) / 128);
     const int64_t tiles_mid = ((M + 63) / 64) * ((N + 63) / 64);
-    if (!use3m) {
-        // streamlined kernel: 64x64 tiles (4 waves, K chunk 16, two
-        // accumulator sets) when there are enough tiles, else 32x32
+    {
+        // streamlined kernel: 64x64 tiles (4 waves, K chunk 16) when there
+        // are enough tiles, else 32x32
         const GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, C, alpha, beta, sk);
-        const bool done = tiles_mid >= 256 ? launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g, nullptr)
-                                           : launch_pair2<T, 32, 32, 16, 1, 1, false>(s, g, nullptr);
+        const bool done = tiles_mid >= 256 ? launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g)
+                                           : launch_cplx<T, 32, 32, 16, 1, 1>(use3m, s, g);
         if (done) {
             RGF_HIP(hipGetLastError());
             return;
@@ -975,7 +1011,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
 template <typename T>
 static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, int use3m,
                        hipEvent_t stop = nullptr) {
-    if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g0, &g1, nullptr, stop)) {
+    if (!launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g0, &g1, nullptr, stop)) {
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
         if (stop) RGF_HIP(hipEventRecord(stop, s));
     }
@@ -986,7 +1022,7 @@ static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& 
 template <typename T>
 static void cgemm_triple(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, const GemmArgs<T>& g2,
                          int use3m, hipEvent_t stop = nullptr) {
-    if (use3m || !launch_pair2<T, 64, 64, 16, 2, 2, false>(s, g0, &g1, &g2, stop)) {
+    if (!launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g0, &g1, &g2, stop)) {
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g2, nullptr, use3m);
         if (stop) RGF_HIP(hipEventRecord(stop, s));
@@ -1800,7 +1836,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         const int nk = nb_of(k), nj = nb_of(j);
         const GemmArgs<T> g = gemm_args<T>(nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk),
                                            cd(-1, 0), 1, Skip{});
-        if (use3m || !launch_pair2<T, 32, 32, 16, 1, 1, false>(s, g, nullptr, nullptr, stop)) {
+        if (!launch_cplx<T, 32, 32, 16, 1, 1>(use3m, s, g, nullptr, nullptr, stop)) {
             cgemm_skip<T>(s, nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk), cd(-1, 0), 1,
                           Skip{}, use3m, nullptr, 0);
             if (stop) RGF_HIP(hipEventRecord(stop, s));

@@ -174,6 +174,23 @@ def test_smatrix_3d(precision, tol):
             assert np.abs(S - Sref).max() < tol
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_mixed_3m_modes(mode):
+    """Three-product complex GEMMs in the slice inverses (bit 0) and/or the
+    refinement sweeps (bit 1): the refined S meets the mixed tolerance against
+    the oracle and FP64 on a 3D problem with several slice-inverse blocks."""
+    rng = np.random.default_rng(31)
+    eps = 1 + 1.5 * rng.random((16, 18, 12))
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.2, k0dx=1.0, xyBC="periodic")
+    S, _, info = rgf_amd.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"], precision="fp32_refined",
+                                            options={"complex_3m": mode, "block_size": 64}, return_info=True)
+    S64, _ = rgf_amd.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"], precision="fp64")
+    Sref, _ = o.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"])
+    assert np.abs(S64 - Sref).max() < TOL64
+    assert np.abs(S - Sref).max() < TOL_MIXED
+    assert info["refine_residual"] <= 1e-13
+
+
 def _cfg1(seed=1):
     rng = np.random.default_rng(seed)
     eps = 1 + 3 * rng.random((64, 200))
