@@ -455,11 +455,12 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         if (!std::isfinite(rel)) throw Error(RGF_ERR_NUMERICAL, "non-finite residual in refinement");
         if (rel <= tol || it >= maxit) break;
         const double sc = 1.0 / rmax;
-        // forward sweep: d_n = g_n (r_n + d_{n-1})
+        // forward sweep: d_n = g_n t_n, t_0 = r_0, t_{n+1} = r_{n+1} + d_n
+        // (scaled by sc; each product's reduction also forms the next t)
+        convert<float, double>(s, N, m, P(r, 0), t, sc);
         for (int64_t n = 0; n < S; ++n) {
-            convert<float, double>(s, N, m, P(r, n), t, sc);
-            if (n > 0) add_mat<float>(s, N, m, t, D(n - 1), t);
-            cgemm<float>(s, N, m, N, Fn(n), t, D(n), cd(1, 0), 0, 0, 0, sweep3m, wk.first, wk.second);
+            const CMat<double> rn = P(r, n + 1);
+            sweep_forward(s, N, m, Fn(n), t, D(n), n + 1 < S ? &rn : nullptr, sc, sweep3m, wk.first, wk.second);
         }
         // backward sweep: d_n += g_n d_{n+1}
         for (int64_t n = S - 2; n >= 0; --n)

@@ -123,6 +123,12 @@ void add_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> a, CMat<T> b, CM
 void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2,
                 const double* g, CMat<double> P1, CMat<double> P2);
 
+// One forward-sweep product of the mixed refinement: D = G t (FP32; G N x N,
+// t and D N x m); with `next`, also t = (float)(sc * next) + D, the right-hand
+// side of the following slice (fused into the split-K reduction).
+void sweep_forward(hipStream_t s, int64_t N, int64_t m, CMat<float> G, CMat<float> t, CMat<float> D,
+                   const CMat<double>* next, double sc, int use3m, float* ws, size_t ws_elems);
+
 // Residual of the block-tridiagonal system for slices 0..L+1 (FP64):
 // r_n = b_n - (A0 - k2 diag(eps_n)) psi_n + psi_{n-1} + psi_{n+1}, with m
 // columns; b_0 holds B_L in columns [0, mL), b_{L+1} holds B_R in columns

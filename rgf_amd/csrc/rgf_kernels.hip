@@ -923,6 +923,20 @@ static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T>
     launch_pair<T, BM, BN, BK, WM, WN>(s, g, nullptr, use3m);
 }
 
+// K chunks for a split-K product with `tiles` 64x64 output tiles: about
+// 1536 work groups in one launch (measured better than one or two full
+// rounds of resident groups), chunks of at least 256; 0 = no split.
+template <typename T>
+static int64_t splitk_chunks(int64_t tiles, int64_t K, size_t ws_elems, int64_t per, int64_t& kc_out) {
+    constexpr int BK = GemmCfg<T>::BK_SMALL;
+    int64_t nz = std::min<int64_t>((1536 + tiles - 1) / tiles, K / 256);
+    nz = std::min<int64_t>(nz, (int64_t)(ws_elems / per));
+    if (nz < 2) return 0;
+    kc_out = ((K + nz - 1) / nz + BK - 1) / BK * BK;
+    nz = (K + kc_out - 1) / kc_out;
+    return nz >= 2 ? nz : 0;
+}
+
 // C = alpha * sum_z P_z + beta * C over the split-K partial products
 template <typename T>
 __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int nz, const T* P, int64_t pstride, T* Cre, T* Cim,
@@ -957,12 +971,9 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
         const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
         if (tiles < 512) {
             const int64_t per = 2 * M * N;  // elements of one partial (two planes)
-            int64_t nz = std::min<int64_t>((1536 + tiles - 1) / tiles, K / 256);
-            nz = std::min<int64_t>(nz, (int64_t)(ws_elems / per));
+            int64_t kc = 0;
+            const int64_t nz = splitk_chunks<T>(tiles, K, ws_elems, per, kc);
             if (nz >= 2) {
-                constexpr int BK = GemmCfg<T>::BK_SMALL;
-                const int64_t kc = ((K + nz - 1) / nz + BK - 1) / BK * BK;
-                nz = (K + kc - 1) / kc;
                 // all chunks in one launch; chunk z writes partial z of the workspace
                 GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, CMat<T>(ws, ws + M * N, N), cd(1, 0), 0, Skip{});
                 g.nz = (int)nz;
@@ -1043,6 +1054,59 @@ template <typename T>
 void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B, CMat<T> C, cd alpha,
            int beta, int64_t skip_lo, int64_t skip_hi, int use3m, T* ws, size_t ws_elems) {
     cgemm_skip<T>(s, M, N, K, A, B, C, alpha, beta, Skip{skip_lo, skip_hi, skip_lo, skip_hi}, use3m, ws, ws_elems);
+}
+
+// Split-K reduction of one forward-sweep product: D = sum_z P_z and, with
+// X given, T = (float)(sc * X) + D (the next right-hand side; the same
+// roundings as a conversion followed by an FP32 addition).
+__global__ void sweep_reduce_kernel(int64_t M, int64_t N, int nz, const float* P, int64_t pstride, float* Dre,
+                                    float* Dim, int64_t ldd, const double* Xre, const double* Xim, int64_t ldx,
+                                    double sc, float* Tre, float* Tim, int64_t ldt) {
+    const int64_t total = M * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        double sr = 0, si = 0;
+        for (int z = 0; z < nz; ++z) {
+            sr += (double)P[z * pstride + e];
+            si += (double)P[z * pstride + total + e];
+        }
+        const int64_t i = e / N, j = e % N;
+        const float dr = (float)sr, di = (float)si;
+        Dre[i * ldd + j] = dr;
+        Dim[i * ldd + j] = di;
+        if (Xre) {
+            Tre[i * ldt + j] = (float)(Xre[i * ldx + j] * sc) + dr;
+            Tim[i * ldt + j] = (float)(Xim[i * ldx + j] * sc) + di;
+        }
+    }
+}
+
+void sweep_forward(hipStream_t s, int64_t N, int64_t m, CMat<float> G, CMat<float> t, CMat<float> D,
+                   const CMat<double>* next, double sc, int use3m, float* ws, size_t ws_elems) {
+    if (N <= 0 || m <= 0) return;
+    // split-K as cgemm_skip would choose it, with the reduction fused
+    const int64_t tiles = ((N + 63) / 64) * ((m + 63) / 64);
+    const int64_t per = 2 * N * m;
+    int64_t nz = 0, kc = 0;
+    if (ws && N >= 1024 && tiles < 512) nz = splitk_chunks<float>(tiles, N, ws_elems, per, kc);
+    if (nz >= 2) {
+        GemmArgs<float> g = gemm_args<float>(N, m, N, G, t, CMat<float>(ws, ws + N * m, m), cd(1, 0), 0, Skip{});
+        g.nz = (int)nz;
+        g.kc = (int)kc;
+        g.zstride = per;
+        if (launch_cplx<float, 64, 64, 16, 2, 2>(use3m, s, g)) {
+            hipLaunchKernelGGL(sweep_reduce_kernel, dim3(grid_for(N * m)), dim3(256), 0, s, N, m, (int)nz,
+                               (const float*)ws, per, D.re, D.im, D.ld, next ? next->re : nullptr,
+                               next ? next->im : nullptr, next ? next->ld : 0, sc, t.re, t.im, t.ld);
+            RGF_HIP(hipGetLastError());
+            return;
+        }
+    }
+    cgemm<float>(s, N, m, N, G, t, D, cd(1, 0), 0, 0, 0, use3m, ws, ws_elems);
+    if (next) {
+        convert<float, double>(s, N, m, *next, t, sc);
+        add_mat<float>(s, N, m, t, D, t);
+    }
 }
 
 // ------------------------------------------------------------------ K2: diagonal-block inverse
@@ -2073,12 +2137,13 @@ __global__ void residual_kernel(int64_t N, int64_t L, int64_t m, const double* p
                                 double* ri, const double* er, const double* ei, double eLr, double eLi, double eRr,
                                 double eRi, double k2r, double k2i, const int32_t* rowptr, const int32_t* col,
                                 const double* vre, const double* vim, const double* b0r, const double* b0i,
-                                int64_t mL, const double* b1r, const double* b1i, int64_t mR) {
+                                int64_t mL, const double* b1r, const double* b1i, int64_t mR, int64_t n0) {
     const int64_t S = N * m;
-    const int64_t total = (L + 2) * S;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = e / S, rem = e % S, i = rem / m, c = rem % m;
+    const int64_t n = n0 + blockIdx.y;  // slice
+    const unsigned um = (unsigned)m;
+    for (unsigned rem = blockIdx.x * blockDim.x + threadIdx.x; rem < (unsigned)S; rem += gridDim.x * blockDim.x) {
+        const unsigned i = rem / um, c = rem - i * um;
+        const int64_t e = n * S + rem;
         // eps of slice n: leads at 0 and L+1, scattering slices 1..L
         double xr, xi;
         if (n == 0) {
@@ -2128,10 +2193,14 @@ void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const doub
                       double* r_re, double* r_im, const double* eps_re, const double* eps_im, cd epsL, cd epsR,
                       cd k2, const DevCSR& A0, const double* b0_re, const double* b0_im, int64_t mL,
                       const double* b1_re, const double* b1_im, int64_t mR) {
-    const int64_t total = (L + 2) * N * m;
-    hipLaunchKernelGGL(residual_kernel, dim3(grid_for(total)), dim3(256), 0, s, N, L, m, psi_re, psi_im, r_re, r_im,
-                       eps_re, eps_im, epsL.real(), epsL.imag(), epsR.real(), epsR.imag(), k2.real(), k2.imag(),
-                       A0.rowptr, A0.col, A0.vre, A0.vim, b0_re, b0_im, mL, b1_re, b1_im, mR);
+    // grid: x over the N*m entries of a slice, y over up to 65535 slices
+    if ((uint64_t)N * (uint64_t)m >= (1ull << 31)) throw Error(RGF_ERR_INVALID_ARGUMENT, "slice too large");
+    for (int64_t n0 = 0; n0 < L + 2; n0 += 65535) {
+        const dim3 grid((unsigned)grid_for(N * m, 256, 1024), (unsigned)std::min<int64_t>(65535, L + 2 - n0));
+        hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, s, N, L, m, psi_re, psi_im, r_re, r_im, eps_re,
+                           eps_im, epsL.real(), epsL.imag(), epsR.real(), epsR.imag(), k2.real(), k2.imag(),
+                           A0.rowptr, A0.col, A0.vre, A0.vim, b0_re, b0_im, mL, b1_re, b1_im, mR, n0);
+    }
     RGF_HIP(hipGetLastError());
 }
 
