@@ -176,6 +176,20 @@ rgf_status rgf_build_laplacian_1d(int64_t n, int32_t bc, double ka, double* A_ou
 rgf_status rgf_solve(rgf_handle h, const rgf_problem* p, const rgf_selection* sel,
                      const rgf_options* opt, double* S_out, rgf_info* info);
 
+/* rgf_solve plus the field profiles: the total field in every slice for each
+ * selected input (no counterpart in the reference, whose recursion never
+ * forms them).  fields_out: N x (L + 2) x n_in complex, interleaved,
+ * column-major, caller-allocated; slice 0 is the left lead's slice next to
+ * the region, slices 1..L are the reference's slices (epsilon's slice
+ * order), slice L + 1 is the right lead's; inputs are ordered as S's
+ * columns; transverse sites as in epsilon.  fields[:, n, a] = G(n, source_a)
+ * B_a, so S = -2i C fields[:, boundary, :] - delta.  Requires precision
+ * RGF_PRECISION_FP32_REFINED: the fields are the refined FP64 solution of the
+ * whole slab, within refine_tol of the FP64 residual. */
+rgf_status rgf_solve_fields(rgf_handle h, const rgf_problem* p, const rgf_selection* sel,
+                            const rgf_options* opt, double* S_out, double* fields_out,
+                            rgf_info* info);
+
 /* Many independent solves (frequencies, Bloch phases, widths) in one call:
  * up to opt->batch_concurrency solves run at once, each on its own worker
  * context (streams and device workspace, kept by the handle for later calls).

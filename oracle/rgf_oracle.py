@@ -507,8 +507,12 @@ def cal_smatrix_RGF_3D(syst: dict, out, in_):
 
 
 def brute_force_smatrix(slices, A0, k0dx2, phi, chL, chR, epsL, epsR,
-                        ind_in_L, ind_in_R, ind_out_L, ind_out_R, G0_L=None, G0_R=None):
+                        ind_in_L, ind_in_R, ind_out_L, ind_out_R, G0_L=None, G0_R=None, return_fields=False):
     """Full S from one sparse solve of the whole block-tridiagonal system.
+
+    With ``return_fields`` also returns the solution itself, the field
+    profiles: an N x (L+2) x n_in array (slice n of input a is
+    ``Psi[:, n, a]``), inputs L then R.
 
     Slices 0..L+1; slice 0 and L+1 are the lead surfaces with diagonal blocks
     A_lead - Sigma where Sigma = G0 is the lead self-energy (G0 satisfies
@@ -545,7 +549,10 @@ def brute_force_smatrix(slices, A0, k0dx2, phi, chL, chR, epsL, epsR,
     P0 = Psi[:N]
     P1 = Psi[N * (L + 1):]
     S = -2j * np.vstack([C_L @ P0, C_R @ P1])
-    return S - _delta(ind_out_L, ind_in_L, ind_out_R, ind_in_R, S.shape)
+    S = S - _delta(ind_out_L, ind_in_L, ind_out_R, ind_in_R, S.shape)
+    if return_fields:
+        return S, Psi.reshape(L + 2, N, m).transpose(1, 0, 2)
+    return S
 
 
 def transmission_gradient_fd(syst: dict, pixels, h: float = 1e-6, three_d: bool = False):

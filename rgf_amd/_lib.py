@@ -93,6 +93,8 @@ SIGNATURES = {
     "rgf_build_laplacian_1d": (C.c_int, [C.c_int64, C.c_int32, C.c_double, _dp]),
     "rgf_solve": (C.c_int, [C.c_void_p, C.POINTER(Problem), C.POINTER(Selection), C.POINTER(Options), _dp,
                             C.POINTER(Info)]),
+    "rgf_solve_fields": (C.c_int, [C.c_void_p, C.POINTER(Problem), C.POINTER(Selection), C.POINTER(Options), _dp,
+                                   _dp, C.POINTER(Info)]),
     "rgf_solve_batched": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Problem), C.POINTER(Selection),
                                     C.POINTER(Options), C.POINTER(_dp), C.POINTER(Info)]),
     "rgf_transmission_gradient": (C.c_int, [C.c_void_p, C.POINTER(Problem), C.POINTER(Selection),

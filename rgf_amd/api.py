@@ -222,45 +222,74 @@ def _options(precision="fp64", **kw) -> L.Options:
     return o
 
 
-def _solve(prob: _Prob, out, in_, precision, device, opts):
+def _solve(prob: _Prob, out, in_, precision, device, opts, fields: bool = False):
+    """S (n_out x n_in) and the info dict; with ``fields``, also the field
+    profiles as an N x (L+2) x n_in array (rgf_solve_fields)."""
     lib = L.load_library()
     nprop = _query(prob)
     sel = _Sel(_parse_selection(out, "out", nprop), _parse_selection(in_, "in", nprop))
     nout, nin = C.c_int32(), C.c_int32()
     L.check(lib.rgf_query(C.byref(prob.p), C.byref(sel.s), None, None, C.byref(nout), C.byref(nin)))
+    N = int(np.prod(prob.p.dims[:prob.p.ndim - 1]))
+    nslab = int(prob.p.dims[prob.p.ndim - 1]) + 2
     if nout.value == 0 or nin.value == 0:
-        return np.zeros((nout.value, nin.value)), None      # cal_smatrix_RGF.m:181-184 (real zeros)
+        S0 = np.zeros((nout.value, nin.value))                # cal_smatrix_RGF.m:181-184 (real zeros)
+        if fields:
+            return S0, None, np.zeros((N, nslab, nin.value), dtype=np.complex128, order="F")
+        return S0, None
     h = L.handle(device)
     S = np.zeros((nout.value, nin.value), dtype=np.complex128, order="F")
     info = L.Info()
     o = _options(precision, **(opts or {}))
+    if fields:
+        F = np.zeros((N, nslab, nin.value), dtype=np.complex128, order="F")
+        st = lib.rgf_solve_fields(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp),
+                                  F.ctypes.data_as(L._dp), C.byref(info))
+        L.check(st, h)
+        return S, info.as_dict(), F
     st = lib.rgf_solve(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp), C.byref(info))
     L.check(st, h)
     return S, info.as_dict()
 
 
 def cal_smatrix_RGF(syst: dict, out, in_, *, precision: str = "fp64", device: int = -1, options: dict | None = None,
-                    return_info: bool = False):
+                    return_info: bool = False, return_fields: bool = False):
     """[smatrix, channels] = cal_smatrix_RGF(syst, out, in) (cal_smatrix_RGF.m:1-291).
 
     Unlike the reference's validation at cal_smatrix_RGF.m:89, a numeric
     ``yBC`` (Bloch phase ky*a, as its help text :32-34 documents) is accepted.
     ``precision`` is 'fp64' (complex FP64 throughout) or 'fp32_refined'.
+    ``return_fields`` (precision 'fp32_refined' only) appends the field
+    profiles, an ``ny x (nx + 2) x n_in`` array: column ``n`` of slice
+    ``1..nx`` lines up with ``epsilon[:, n - 1]``, slices 0 and nx+1 are the
+    lead slices next to the region, inputs in S's column order.
     """
     prob = _problem_2d(syst)
-    S, info = _solve(prob, out, in_, precision, device, options)
+    res = _solve(prob, out, in_, precision, device, options, fields=return_fields)
+    S, info = res[0], res[1]
     ch = setup_channels(prob.p.dims[0], syst["yBC"], syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"])
-    return (S, ch, info) if return_info else (S, ch)
+    ret = (S, ch, info) if return_info else (S, ch)
+    return ret + (res[2],) if return_fields else ret
 
 
 def cal_smatrix_RGF_3D(syst: dict, out, in_, *, precision: str = "fp64", device: int = -1,
-                       options: dict | None = None, return_info: bool = False):
-    """[smatrix, channels] = cal_smatrix_RGF_3D(syst, out, in) (cal_smatrix_RGF_3D.m:1-306)."""
+                       options: dict | None = None, return_info: bool = False, return_fields: bool = False):
+    """[smatrix, channels] = cal_smatrix_RGF_3D(syst, out, in) (cal_smatrix_RGF_3D.m:1-306).
+
+    ``return_fields`` (precision 'fp32_refined' only) appends the field
+    profiles, an ``nx x ny x (nz + 2) x n_in`` array aligned with
+    ``epsilon`` in slices 1..nz (slices 0 and nz+1 are the lead slices).
+    """
     prob = _problem_3d(syst)
-    S, info = _solve(prob, out, in_, precision, device, options)
+    res = _solve(prob, out, in_, precision, device, options, fields=return_fields)
+    S, info = res[0], res[1]
     nx, ny = prob.p.dims[0], prob.p.dims[1]
     ch = setup_channels_3D(nx, ny, syst["xyBC"], syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"])
-    return (S, ch, info) if return_info else (S, ch)
+    ret = (S, ch, info) if return_info else (S, ch)
+    if not return_fields:
+        return ret
+    F = res[2]
+    return ret + (F.reshape((nx, ny, F.shape[1], F.shape[2]), order="F"),)
 
 
 def cal_smatrix_RGF_batched(items, *, precision: str = "fp64", device: int = -1, options: dict | None = None,

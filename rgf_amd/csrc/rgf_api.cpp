@@ -168,6 +168,7 @@ struct Work {
     KernelTimer* timer = nullptr;
     int refine_iters = 0;
     double refine_residual = 0;
+    double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
 };
 
 void upload_eps(rgf_context* c, const rgf_problem* p, Work& w) {
@@ -494,6 +495,27 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
             for (int64_t j = 0; j < mR; ++j) w.TRR[i * mR + j] = h[i * m + mL + j];
         }
     }
+    if (w.fields) {
+        // field profiles in the caller's order: inputs L then R, physical
+        // slices 0..L+1 (the sweep may run mirrored)
+        const int64_t nin = mL + mR;
+        const int64_t nIL = w.mirrored ? mR : mL;  // physical left inputs
+        std::vector<int32_t> col((size_t)nin);
+        for (int64_t a = 0; a < nin; ++a) {
+            const bool physR = a >= nIL;
+            const int64_t k = physR ? a - nIL : a;
+            const bool sweepL = w.mirrored ? physR : !physR;
+            col[(size_t)a] = (int32_t)(sweepL ? k : mL + k);
+        }
+        int32_t* dcol = dbuf<int32_t>(c, "field_col", (size_t)nin);
+        RGF_HIP(hipMemcpyAsync(dcol, col.data(), nin * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        const size_t cnt = (size_t)2 * N * (L + 2) * nin;
+        double* pk = dbuf<double>(c, "field_pack", cnt);
+        pack_fields(s, N, L, m, nin, psi, psi + S * SE, dcol, w.mirrored ? 1 : 0, pk);
+        RGF_HIP(hipMemcpyAsync(w.fields, pk, cnt * sizeof(double), hipMemcpyDeviceToHost, s));
+        RGF_HIP(hipStreamSynchronize(s));
+        release(c, "field_pack");
+    }
     w.t_refine = now() - t1;
     if (rel > tol) {
         char buf[160];
@@ -729,7 +751,7 @@ void write_smatrix(const Work& w, const Selection& sel, double* S_out) {
 }
 
 void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt, double* S_out,
-               rgf_info* info) {
+               rgf_info* info, double* fields = nullptr) {
     const double t0 = now();
     rgf_options o;
     rgf_default_options(&o);
@@ -740,6 +762,9 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
     Selection sel = make_selection(s, ch);
     fill_info(info, ch, sel);
     const int64_t nout = sel.outL.size() + sel.outR.size(), nin = sel.inL.size() + sel.inR.size();
+    if (fields && o.precision != RGF_PRECISION_FP32_REFINED)
+        throw Error(RGF_ERR_INVALID_ARGUMENT,
+                    "field profiles come from the refined solution: use precision RGF_PRECISION_FP32_REFINED");
     if (nout == 0 || nin == 0) return;  // cal_smatrix_RGF.m:181-184
     if (!S_out) throw Error(RGF_ERR_INVALID_ARGUMENT, "S_out is NULL");
     RGF_HIP(hipSetDevice(c->device));
@@ -750,6 +775,7 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
         w.timer = timer.get();
     }
     prepare(c, p, ch, sel, w);
+    w.fields = fields;
     w.t_setup = now() - t0;
     if (o.precision == RGF_PRECISION_FP64)
         run_carry(c, w, o);
@@ -982,6 +1008,15 @@ rgf_status rgf_solve(rgf_handle h, const rgf_problem* p, const rgf_selection* s,
     return guard(h, [&] {
         if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
         solve_one(h, p, s, opt, S_out, info);
+    });
+}
+
+rgf_status rgf_solve_fields(rgf_handle h, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt,
+                            double* S_out, double* fields_out, rgf_info* info) {
+    return guard(h, [&] {
+        if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
+        if (!fields_out) throw Error(RGF_ERR_INVALID_ARGUMENT, "fields_out is NULL");
+        solve_one(h, p, s, opt, S_out, info, fields_out);
     });
 }
 

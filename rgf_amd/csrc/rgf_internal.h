@@ -129,6 +129,12 @@ void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const d
 void sweep_forward(hipStream_t s, int64_t N, int64_t m, CMat<float> G, CMat<float> t, CMat<float> D,
                    const CMat<double>* next, double sc, int use3m, float* ws, size_t ws_elems);
 
+// Field profiles from the refined solution psi ((L+2) slices of N x m planar
+// FP64): out (device, interleaved complex, column-major N x (L+2) x nin) takes
+// column col[a] for input a, slices in physical order (reversed if mirrored).
+void pack_fields(hipStream_t s, int64_t N, int64_t L, int64_t m, int64_t nin, const double* psi_re,
+                 const double* psi_im, const int32_t* col, int mirrored, double* out);
+
 // Residual of the block-tridiagonal system for slices 0..L+1 (FP64):
 // r_n = b_n - (A0 - k2 diag(eps_n)) psi_n + psi_{n-1} + psi_{n+1}, with m
 // columns; b_0 holds B_L in columns [0, mL), b_{L+1} holds B_R in columns

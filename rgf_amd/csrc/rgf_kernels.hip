@@ -2204,6 +2204,33 @@ void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const doub
     RGF_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ field profiles
+// out[(i + N * np) + N * (L + 2) * a] (interleaved complex) = psi at physical
+// slice np, transverse site i, input a: internal slice (mirrored ? L+1-np : np),
+// internal column col[a].
+__global__ void pack_fields_kernel(int64_t N, int64_t L, int64_t m, int64_t nin, const double* pr, const double* pim,
+                                   const int32_t* col, int mirrored, double* out) {
+    const int64_t per = N * (L + 2);
+    const int64_t total = per * nin;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = e / per, rem = e - a * per, np = rem / N, i = rem - np * N;
+        const int64_t n = mirrored ? L + 1 - np : np;
+        const int64_t src = (n * N + i) * m + col[a];
+        out[2 * e] = pr[src];
+        out[2 * e + 1] = pim[src];
+    }
+}
+
+void pack_fields(hipStream_t s, int64_t N, int64_t L, int64_t m, int64_t nin, const double* psi_re,
+                 const double* psi_im, const int32_t* col, int mirrored, double* out) {
+    const int64_t total = N * (L + 2) * nin;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(pack_fields_kernel, dim3(grid_for(total)), dim3(256), 0, s, N, L, m, nin, psi_re, psi_im, col,
+                       mirrored, out);
+    RGF_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ adjoint contraction
 __global__ void grad_contract_kernel(int64_t N, int64_t m, const double* pr, const double* pim, int64_t ldp,
                                      const double* fr, const double* fi, int64_t ldf, double k2r, double k2i,

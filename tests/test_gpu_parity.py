@@ -191,6 +191,65 @@ def test_mixed_3m_modes(mode):
     assert info["refine_residual"] <= 1e-13
 
 
+def _fields_ref_2d(syst, out, inn):
+    """Oracle fields by the brute-force full-system solve (2D)."""
+    eps = np.asarray(syst["epsilon"])
+    ny, nx = eps.shape
+    S, ch = o.cal_smatrix_RGF(syst, out, inn)
+    iL = np.arange(1, ch["L"]["N_prop"] + 1)
+    iR = np.arange(1, ch["R"]["N_prop"] + 1)
+    sel_in = {"L": iL, "R": iR}
+    inL = sel_in["L"] if "L" in inn else np.zeros(0, dtype=int)
+    inR = sel_in["R"] if "R" in inn else np.zeros(0, dtype=int)
+    A0 = o.slice_operator_2d(ny, syst["yBC"])
+    Sb, F = o.brute_force_smatrix([eps[:, n] for n in range(nx)], A0, syst["k0dx"] ** 2, ch["phi"], ch["L"],
+                                  ch["R"], syst["epsilon_L"], syst["epsilon_R"], inL, inR, iL, iR,
+                                  return_fields=True)
+    return F
+
+
+@pytest.mark.parametrize("bc,inn", [("periodic", ["L", "R"]), ("Dirichlet", ["R"]), (0.9, ["L"])])
+def test_field_profiles_2d(bc, inn):
+    """Fields of the refined solve against the brute-force full-system solve
+    (with inputs ["L"] only, the sweep runs mirrored: more left selections)."""
+    rng = np.random.default_rng(41)
+    eps = 1 + 2 * rng.random((14, 11))
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.5, k0dx=1.1, yBC=bc)
+    S, ch, info, F = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], inn, precision="fp32_refined", return_info=True,
+                                             return_fields=True)
+    Fref = _fields_ref_2d(syst, ["L", "R"], inn)
+    assert F.shape == Fref.shape == (14, 13, S.shape[1])
+    assert np.abs(F - Fref).max() <= 1e-9 * np.abs(Fref).max()
+    Sref, _ = o.cal_smatrix_RGF(syst, ["L", "R"], inn)
+    assert np.abs(S - Sref).max() < TOL_MIXED
+    assert info["swept_right_to_left"] == (inn == ["L"])
+
+
+def test_field_profiles_3d():
+    rng = np.random.default_rng(42)
+    nx, ny, nz = 5, 4, 6
+    eps = 1 + rng.random((nx, ny, nz))
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.3, k0dx=1.2, xyBC=[0.4, 2.2])
+    S, ch, F = rgf_amd.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"], precision="fp32_refined",
+                                          return_fields=True)
+    Sref, chr_ = o.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"])
+    iL = np.arange(1, chr_["L"]["N_prop"] + 1)
+    iR = np.arange(1, chr_["R"]["N_prop"] + 1)
+    A0 = o.slice_operator_3d(nx, ny, [0.4, 2.2])
+    _, Fref = o.brute_force_smatrix([eps[:, :, n].ravel(order="F") for n in range(nz)], A0, 1.2 ** 2,
+                                    chr_["phi"], chr_["L"], chr_["R"], 1.0, 1.3, iL, iR, iL, iR, return_fields=True)
+    assert F.shape == (nx, ny, nz + 2, S.shape[1])
+    Fr = Fref.reshape((nx, ny, nz + 2, S.shape[1]), order="F")
+    assert np.abs(F - Fr).max() <= 1e-9 * np.abs(Fr).max()
+    assert np.abs(S - Sref).max() < TOL_MIXED
+
+
+def test_field_profiles_need_refined_precision():
+    syst = dict(epsilon=np.ones((6, 4)), epsilon_L=1.0, epsilon_R=1.0, k0dx=1.0, yBC="periodic")
+    with pytest.raises(ValueError, match="refined"):
+        rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"], precision="fp64", return_fields=True)
+
+
 def _cfg1(seed=1):
     rng = np.random.default_rng(seed)
     eps = 1 + 3 * rng.random((64, 200))

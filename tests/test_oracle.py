@@ -184,6 +184,33 @@ def test_recursion_vs_brute_force_2d(bc, nx):
     assert np.abs(S - Sb).max() < 1e-12
 
 
+def test_brute_force_fields_satisfy_slice_equations():
+    """The field profiles returned by the brute-force solve: interior slices
+    obey (A0 - k^2 eps_n) psi_n = psi_{n-1} + psi_{n+1}, the lead slices carry
+    the source and self-energy, and S follows from the boundary slices."""
+    rng = np.random.default_rng(8)
+    ny, nx, k0dx = 9, 5, 2 * np.pi / 5.5
+    syst = _full_2d(rng, ny, nx, "periodic", 1.0, 2.3, k0dx)
+    S, ch = o.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"])
+    iL = np.arange(1, ch["L"]["N_prop"] + 1)
+    iR = np.arange(1, ch["R"]["N_prop"] + 1)
+    A0 = o.slice_operator_2d(ny, "periodic")
+    eps = syst["epsilon"]
+    Sb, F = o.brute_force_smatrix([eps[:, n] for n in range(nx)], A0, k0dx ** 2, ch["phi"], ch["L"], ch["R"],
+                                  1.0, 2.3, iL, iR, iL, iR, return_fields=True)
+    assert F.shape == (ny, nx + 2, len(iL) + len(iR))
+    assert np.abs(S - Sb).max() < 1e-12
+    for n in range(1, nx + 1):
+        lhs = (A0 - np.diag(k0dx ** 2 * eps[:, n - 1])) @ F[:, n]
+        assert np.abs(lhs - F[:, n - 1] - F[:, n + 1]).max() < 1e-10 * np.abs(F).max()
+    # left lead slice: (A_lead - G0_L) psi_0 - psi_1 = B_L (left inputs), 0 (right)
+    G0L = o.lead_greens_function(ch["phi"], ch["L"]["kxdx"])
+    BL = ch["phi"][:, ch["L"]["ind_prop"][iL - 1] - 1] * ch["L"]["sqrt_mu"][iL - 1][None, :]
+    r0 = (A0 - k0dx ** 2 * 1.0 * np.eye(ny) - G0L) @ F[:, 0] - F[:, 1]
+    assert np.abs(r0[:, :len(iL)] - BL).max() < 1e-10 * np.abs(F).max()
+    assert np.abs(r0[:, len(iL):]).max() < 1e-10 * np.abs(F).max()
+
+
 def test_five_schemes_agree_with_general():
     rng = np.random.default_rng(3)
     syst = _full_2d(rng, 10, 7, "periodic", 1.2, 2.0, 2 * np.pi / 6)
