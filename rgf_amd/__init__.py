@@ -16,9 +16,10 @@ from .api import (  # noqa: F401
     setup_channels_3D,
     transmission_gradient,
 )
+from .sweep import load_scan, save_scan, scan  # noqa: F401
 
 __all__ = [
     "cal_smatrix_RGF", "cal_smatrix_RGF_3D", "setup_channels", "setup_channels_3D",
     "build_laplacian_1d", "load_library", "RGFError", "dense_inverse", "dense_gemm",
-    "transmission_gradient", "cal_smatrix_RGF_batched",
+    "transmission_gradient", "cal_smatrix_RGF_batched", "scan", "save_scan", "load_scan",
 ]

@@ -436,9 +436,9 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     auto P = [&](double* base, int64_t n) { return CMat<double>(base + n * SE, base + S * SE + n * SE, m); };
     auto D = [&](int64_t n) { return CMat<float>(d + n * SE, d + S * SE + n * SE, m); };
     RGF_HIP(hipMemsetAsync(psi, 0, 2 * S * SE * sizeof(double), s));
-    double bmax = std::max(max_abs(s, N * mL, w.BL.re, w.BL.im, c->d_scratch),
-                           max_abs(s, N * mR, w.BR.re, w.BR.im, c->d_scratch));
-    if (bmax == 0) bmax = 1;
+    const double b0 = std::max(max_abs(s, N * mL, w.BL.re, w.BL.im, c->d_scratch),
+                               max_abs(s, N * mR, w.BR.re, w.BR.im, c->d_scratch));
+    const double bmax = b0 == 0 ? 1.0 : b0;
     auto wk = splitk_ws<float>(c);
     auto wk64 = splitk_ws<double>(c);
     const int maxit = o.max_refine_iter > 0 ? o.max_refine_iter : 6;
@@ -446,12 +446,33 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     double rel = 1.0;
     int it = 0;
     for (;; ++it) {
-        // r = b - H psi
-        residual_stencil(s, N, L, m, psi, psi + S * SE, r, r + S * SE, w.eps_re, w.eps_im, w.epsL, w.epsR, w.k2, w.A0,
-                         w.BL.re, w.BL.im, mL, w.BR.re, w.BR.im, mR);
-        cgemm<double>(s, N, m, N, w.G0L, P(psi, 0), P(r, 0), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
-        cgemm<double>(s, N, m, N, w.G0R, P(psi, L + 1), P(r, L + 1), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
-        const double rmax = max_abs(s, S * SE, r, r + S * SE, c->d_scratch);
+        double rmax;
+        if (it == 0) {
+            // psi = 0: r = b (B_L in slice 0, B_R in slice L+1), max|r| = bmax
+            RGF_HIP(hipMemsetAsync(r, 0, 2 * S * SE * sizeof(double), s));
+            const CMat<double> r0 = P(r, 0), r1 = P(r, L + 1);
+            if (mL) {
+                RGF_HIP(hipMemcpy2DAsync(r0.re, m * sizeof(double), w.BL.re, w.BL.ld * sizeof(double),
+                                         mL * sizeof(double), N, hipMemcpyDeviceToDevice, s));
+                RGF_HIP(hipMemcpy2DAsync(r0.im, m * sizeof(double), w.BL.im, w.BL.ld * sizeof(double),
+                                         mL * sizeof(double), N, hipMemcpyDeviceToDevice, s));
+            }
+            if (mR) {
+                RGF_HIP(hipMemcpy2DAsync(r1.re + mL, m * sizeof(double), w.BR.re, w.BR.ld * sizeof(double),
+                                         mR * sizeof(double), N, hipMemcpyDeviceToDevice, s));
+                RGF_HIP(hipMemcpy2DAsync(r1.im + mL, m * sizeof(double), w.BR.im, w.BR.ld * sizeof(double),
+                                         mR * sizeof(double), N, hipMemcpyDeviceToDevice, s));
+            }
+            rmax = b0;
+        } else {
+            // r = b - H psi
+            residual_stencil(s, N, L, m, psi, psi + S * SE, r, r + S * SE, w.eps_re, w.eps_im, w.epsL, w.epsR, w.k2,
+                             w.A0, w.BL.re, w.BL.im, mL, w.BR.re, w.BR.im, mR);
+            cgemm<double>(s, N, m, N, w.G0L, P(psi, 0), P(r, 0), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
+            cgemm<double>(s, N, m, N, w.G0R, P(psi, L + 1), P(r, L + 1), cd(1, 0), 1, 0, 0, 0, wk64.first,
+                          wk64.second);
+            rmax = max_abs(s, S * SE, r, r + S * SE, c->d_scratch);
+        }
         rel = rmax / bmax;
         if (!std::isfinite(rel)) throw Error(RGF_ERR_NUMERICAL, "non-finite residual in refinement");
         if (rel <= tol || it >= maxit) break;
