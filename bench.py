@@ -18,8 +18,10 @@ of them would add ~10 us of event latency per block step).
 traffic (HBM bytes per launch) and MFMA utilisation of that kernel come from
 the rocprofv3 counter summary profiles/pmc_slice_inverse_n4096.md (FETCH_SIZE
 and WRITE_SIZE scaled by a copy kernel of known size, MfmaUtil).
-The mixed path refines to a relative residual of --refine-tol (1e-10), which
-holds the S error against the FP64 path near 1e-11, inside the metric's 1e-9.
+The mixed path refines to a relative residual of --refine-tol, by default the
+metric's own S tolerance 1e-9 (three correction sweeps here: residual 3.4e-10,
+S error against the FP64 path 2.2e-10); the line reports the measured S error
+and whether it is inside the bound ("s_err_within_bound").
 cpu_baseline: the FP64 oracle (numpy/LAPACK) general-scheme slice update at
 the same N, timed on the host cores for a bounded number of slices.
 """
@@ -40,6 +42,7 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 HBM_GBPS = 8000.0
 METRIC = "slice-updates/s, 3D N=4096: FP32+refine vs FP64, S err<=1e-9; MFMA util"
+S_ERR_BOUND = 1e-9
 
 
 def cpu_baseline(syst, n_slices: int = 2):
@@ -132,7 +135,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="slice-inverse panel width (0 = default)")
     ap.add_argument("--single-stream", action="store_true",
                     help="no look-ahead stream in the slice inverse (for hardware-counter runs)")
-    ap.add_argument("--refine-tol", type=float, default=1e-10,
+    ap.add_argument("--refine-tol", type=float, default=1e-9,
                     help="mixed path: stop the FP64 refinement at this relative residual")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -201,6 +204,7 @@ def main():
                        "roofline": roofline(i64, "fp64"), "unitarity_defect": unitarity(S64)}
         rec["ratio_vs_fp64"] = round(value / (L / t64), 3)
         rec["s_err_vs_fp64"] = float(np.abs(S32 - S64).max())
+        rec["s_err_within_bound"] = bool(rec["s_err_vs_fp64"] <= S_ERR_BOUND)
     if not args.skip_cpu:
         rec["cpu_baseline"] = cpu_baseline(syst, args.cpu_slices)
     print(json.dumps(rec), flush=True)
