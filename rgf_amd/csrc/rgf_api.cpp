@@ -410,12 +410,14 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     RGF_HIP(hipStreamSynchronize(s));
     const double t0 = now();
     for (int64_t n = 1; n <= L + 1; ++n) {
-        copy_mat<float>(s, N, N, Fn(n - 1), Fn(n));
-        if (n <= L)
+        const CMat<float> prev = Fn(n - 1);
+        if (n <= L) {
             assemble_slice<float>(s, N, Fn(n), w.eps_re + (n - 1) * N, w.eps_im ? w.eps_im + (n - 1) * N : nullptr,
-                                  cd(0, 0), w.k2, nullptr, w.A0);
-        else
+                                  cd(0, 0), w.k2, nullptr, w.A0, &prev);
+        } else {
+            copy_mat<float>(s, N, N, prev, Fn(n));
             assemble_slice<float>(s, N, Fn(n), nullptr, nullptr, w.epsR, w.k2, &G0R32, w.A0);
+        }
         invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer,
                               o.single_stream ? nullptr : &c->ov);
     }
@@ -586,12 +588,13 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
     double* ws = dbuf<double>(c, "inv_ws", inverse_workspace_elems<double>(N, b));
     auto wk = splitk_ws<double>(c);
     auto dyson = [&](int64_t n, CMat<double> prev, CMat<double> out) {
-        copy_mat<double>(s, N, N, prev, out);
-        if (n <= L)
+        if (n <= L) {
             assemble_slice<double>(s, N, out, w.eps_re + (n - 1) * N, w.eps_im ? w.eps_im + (n - 1) * N : nullptr,
-                                   cd(0, 0), w.k2, nullptr, w.A0);
-        else
+                                   cd(0, 0), w.k2, nullptr, w.A0, &prev);
+        } else {
+            copy_mat<double>(s, N, N, prev, out);
             assemble_slice<double>(s, N, out, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
+        }
         invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer, &c->ov);
     };
     reset_status(c);

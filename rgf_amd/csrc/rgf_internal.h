@@ -96,10 +96,11 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
 
 // M <- A0 - diag(k2*eps) - M - Gextra  (Gextra may be null).  eps is a
 // planar FP64 vector (re, im; im may be null) or, when eps_re is null, the
-// constant eps_const.
+// constant eps_const.  With src, src takes the place of M on the right (M is
+// only written: one pass instead of a copy and an update).
 template <typename T>
 void assemble_slice(hipStream_t s, int64_t N, CMat<T> M, const double* eps_re, const double* eps_im,
-                    cd eps_const, cd k2, const CMat<T>* Gextra, const DevCSR& A0);
+                    cd eps_const, cd k2, const CMat<T>* Gextra, const DevCSR& A0, const CMat<T>* src = nullptr);
 
 // Dense helpers.
 template <typename T>

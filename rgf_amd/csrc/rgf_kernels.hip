@@ -1966,13 +1966,22 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
 template <typename T>
 __global__ void assemble_dense_kernel(int64_t N, T* Mre, T* Mim, int64_t ld, const double* eps_re,
                                       const double* eps_im, double ecr, double eci, double k2r, double k2i,
-                                      const T* Xre, const T* Xim, int64_t ldx) {
+                                      const T* Xre, const T* Xim, int64_t ldx, const T* Sre, const T* Sim,
+                                      int64_t lds) {
     const int64_t total = N * N;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = e / N, j = e % N;
         const int64_t o = i * ld + j;
-        double vr = -(double)Mre[o], vi = -(double)Mim[o];
+        // the previous slice's matrix: in M itself, or in S (M is then only written)
+        double vr, vi;
+        if (Sre) {
+            vr = -(double)Sre[i * lds + j];
+            vi = -(double)Sim[i * lds + j];
+        } else {
+            vr = -(double)Mre[o];
+            vi = -(double)Mim[o];
+        }
         if (Xre) {
             vr -= (double)Xre[i * ldx + j];
             vi -= (double)Xim[i * ldx + j];
@@ -2003,11 +2012,12 @@ __global__ void csr_add_kernel(int32_t n, const int32_t* rowptr, const int32_t* 
 
 template <typename T>
 void assemble_slice(hipStream_t s, int64_t N, CMat<T> M, const double* eps_re, const double* eps_im,
-                    cd eps_const, cd k2, const CMat<T>* Gextra, const DevCSR& A0) {
+                    cd eps_const, cd k2, const CMat<T>* Gextra, const DevCSR& A0, const CMat<T>* src) {
     hipLaunchKernelGGL((assemble_dense_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, M.re, M.im, M.ld,
                        eps_re, eps_im, eps_const.real(), eps_const.imag(), k2.real(), k2.imag(),
                        Gextra ? Gextra->re : (const T*)nullptr, Gextra ? Gextra->im : (const T*)nullptr,
-                       Gextra ? Gextra->ld : (int64_t)0);
+                       Gextra ? Gextra->ld : (int64_t)0, src ? src->re : (const T*)nullptr,
+                       src ? src->im : (const T*)nullptr, src ? src->ld : (int64_t)0);
     hipLaunchKernelGGL((csr_add_kernel<T>), dim3((A0.n + 255) / 256), dim3(256), 0, s, A0.n, A0.rowptr, A0.col,
                        A0.vre, A0.vim, M.re, M.im, M.ld);
     RGF_HIP(hipGetLastError());
@@ -2293,7 +2303,7 @@ double max_abs(hipStream_t s, int64_t count, const double* re, const double* im,
     template void invert_inplace<T>(hipStream_t, int64_t, CMat<T>, int, T*, int*, int, KernelTimer*,           \
                                     const Overlap*);                                                          \
     template void assemble_slice<T>(hipStream_t, int64_t, CMat<T>, const double*, const double*, cd, cd,      \
-                                    const CMat<T>*, const DevCSR&);                                           \
+                                    const CMat<T>*, const DevCSR&, const CMat<T>*);                           \
     template void fill_zero<T>(hipStream_t, int64_t, int64_t, CMat<T>);                                       \
     template void set_identity<T>(hipStream_t, int64_t, CMat<T>);                                             \
     template void copy_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>);                               \

@@ -372,3 +372,24 @@ def test_cfg2_full_size_mixed_vs_fp64():
     for S in (S64, S32):
         assert np.linalg.eigvalsh(S.conj().T @ S).max() <= 1 + 1e-9
     print(f"cfg2: refine iterations {info['refine_iters']}, residual {info['refine_residual']:.2e}", flush=True)
+
+
+def test_cfg3_full_size_mixed_vs_fp64():
+    """3D N = 64 x 64 = 4096, 128 slices, 40 spheres eps 2.1, Bloch random
+    phases, lambda 12 px, full S (182 x 182): the mixed path at the library's
+    tolerance and at the benchmark's (relative residual 1e-9) against the FP64
+    path; all three unitary (real eps, equal leads)."""
+    from rgf_amd.workloads import spheres_3d
+    syst, out, inn = spheres_3d()
+    S64, _ = rgf_amd.cal_smatrix_RGF_3D(syst, out, inn)
+    S32, _, info = rgf_amd.cal_smatrix_RGF_3D(syst, out, inn, precision="fp32_refined", return_info=True)
+    Sb, _, ib = rgf_amd.cal_smatrix_RGF_3D(syst, out, inn, precision="fp32_refined", options={"refine_tol": 1e-9},
+                                           return_info=True)
+    assert S64.shape == (182, 182)
+    assert np.abs(S32 - S64).max() < 1e-11
+    assert np.abs(Sb - S64).max() < TOL_MIXED
+    assert ib["refine_residual"] <= 1e-9 and ib["refine_iters"] <= info["refine_iters"]
+    for S in (S64, S32, Sb):
+        assert np.abs(S.conj().T @ S - np.eye(182)).max() < TOL_MIXED
+    print(f"cfg3: refine {info['refine_iters']} / {ib['refine_iters']} sweeps, "
+          f"S err {np.abs(S32 - S64).max():.2e} / {np.abs(Sb - S64).max():.2e}", flush=True)
