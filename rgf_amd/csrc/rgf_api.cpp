@@ -334,9 +334,10 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     CMat<double> Z = dmat<double>(c, "Z", po, pi);
     auto wk = splitk_ws<double>(c);
     copy_mat<double>(s, N, N, w.G0L, G);
-    if (pi) cgemm<double>(s, N, pi, N, w.G0L, w.BL, X, cd(1, 0), 0);
-    if (po) cgemm<double>(s, po, N, N, w.CL, w.G0L, Y, cd(1, 0), 0);
-    if (po && pi) cgemm<double>(s, po, pi, N, w.CL, X, Z, cd(1, 0), 0);
+    // thin products with K = N: split along K over the workspace
+    if (pi) cgemm<double>(s, N, pi, N, w.G0L, w.BL, X, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
+    if (po) cgemm<double>(s, po, N, N, w.CL, w.G0L, Y, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
+    if (po && pi) cgemm<double>(s, po, pi, N, w.CL, X, Z, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
     reset_status(c);
     RGF_HIP(hipStreamSynchronize(s));
     const double t0 = now();
@@ -366,18 +367,18 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     if (po && pi) w.TLL = download(c, Z, po, pi);
     if (qo && pi) {
         CMat<double> T = dmat<double>(c, "T", qo, pi);
-        cgemm<double>(s, qo, pi, N, w.CR, X, T, cd(1, 0), 0);
+        cgemm<double>(s, qo, pi, N, w.CR, X, T, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
         w.TRL = download(c, T, qo, pi);
     }
     if (po && qi) {
         CMat<double> T = dmat<double>(c, "T", po, qi);
-        cgemm<double>(s, po, qi, N, Y, w.BR, T, cd(1, 0), 0);
+        cgemm<double>(s, po, qi, N, Y, w.BR, T, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
         w.TLR = download(c, T, po, qi);
     }
     if (qo && qi) {
         CMat<double> GB = dmat<double>(c, "GB", N, qi), T = dmat<double>(c, "T", qo, qi);
-        cgemm<double>(s, N, qi, N, G, w.BR, GB, cd(1, 0), 0);
-        cgemm<double>(s, qo, qi, N, w.CR, GB, T, cd(1, 0), 0);
+        cgemm<double>(s, N, qi, N, G, w.BR, GB, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
+        cgemm<double>(s, qo, qi, N, w.CR, GB, T, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
         w.TRR = download(c, T, qo, qi);
     }
 }
@@ -498,7 +499,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     const int64_t po = w.outL.size(), qo = w.outR.size();
     if (po) {
         CMat<double> T = dmat<double>(c, "T", po, m);
-        cgemm<double>(s, po, m, N, w.CL, P(psi, 0), T, cd(1, 0), 0);
+        cgemm<double>(s, po, m, N, w.CL, P(psi, 0), T, cd(1, 0), 0, 0, 0, 0, wk64.first, wk64.second);
         auto h = download(c, T, po, m);
         w.TLL.assign((size_t)po * mL, 0);
         w.TLR.assign((size_t)po * mR, 0);
@@ -509,7 +510,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     }
     if (qo) {
         CMat<double> T = dmat<double>(c, "T", qo, m);
-        cgemm<double>(s, qo, m, N, w.CR, P(psi, L + 1), T, cd(1, 0), 0);
+        cgemm<double>(s, qo, m, N, w.CR, P(psi, L + 1), T, cd(1, 0), 0, 0, 0, 0, wk64.first, wk64.second);
         auto h = download(c, T, qo, m);
         w.TRL.assign((size_t)qo * mL, 0);
         w.TRR.assign((size_t)qo * mR, 0);
@@ -602,7 +603,7 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
     const double t0 = now();
     // forward sweep
     copy_mat<double>(s, N, N, w.G0L, slot_g(0));
-    cgemm<double>(s, N, m, N, w.G0L, w.BL, slot_z(0), cd(1, 0), 0);
+    cgemm<double>(s, N, m, N, w.G0L, w.BL, slot_z(0), cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
     copy_mat<double>(s, N, N, w.G0L, G);
     copy_mat<double>(s, N, m, slot_z(0), za);
     for (int64_t n = 1; n <= L + 1; ++n) {
@@ -616,7 +617,7 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
     }
     // t = -2i C_R z_{L+1}
     CMat<double> Tm = dmat<double>(c, "T", no, m);
-    cgemm<double>(s, no, m, N, w.CR, za, Tm, cd(0, -2), 0);
+    cgemm<double>(s, no, m, N, w.CR, za, Tm, cd(0, -2), 0, 0, 0, 0, wk.first, wk.second);
     std::vector<cd> t = download(c, Tm, no, m);
     T = 0;
     for (auto& v : t) T += std::norm(v);
@@ -628,7 +629,7 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
     upload(c, tH, m, no, TH);
     CMat<double> pha = dmat<double>(c, "pha", m, N), phb = dmat<double>(c, "phb", m, N);
     cgemm<double>(s, m, N, no, TH, w.CR, W, cd(1, 0), 0);
-    cgemm<double>(s, m, N, N, W, G, pha, cd(1, 0), 0);
+    cgemm<double>(s, m, N, N, W, G, pha, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
     CMat<double> psa = dmat<double>(c, "psa", N, m), psb = dmat<double>(c, "psb", N, m);
     copy_mat<double>(s, N, m, za, psa);
     double* dg = dbuf<double>(c, "grad", 2 * (size_t)N * std::max<int64_t>(L, 1));
