@@ -448,6 +448,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     const double tol = o.refine_tol > 0 ? o.refine_tol : 1e-13;
     double rel = 1.0;
     int it = 0;
+    int64_t skipped_cols = 0;  // sweep product columns left out in the first correction
     for (;; ++it) {
         double rmax;
         if (it == 0) {
@@ -481,11 +482,19 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         if (rel <= tol || it >= maxit) break;
         const double sc = 1.0 / rmax;
         // forward sweep: d_n = g_n t_n, t_0 = r_0, t_{n+1} = r_{n+1} + d_n
-        // (scaled by sc; each product's reduction also forms the next t)
+        // (scaled by sc; each product's reduction also forms the next t).  In
+        // the first correction r = b, so t_n and d_n for n < S-2 have only the
+        // left-input columns: those products run on the first mf columns (the
+        // others of d stay zero; t keeps its zero columns until the product at
+        // S-2, which runs in full and adds B_R to the last t)
+        const int64_t mf = it == 0 ? std::min<int64_t>(m, (mL + 3) / 4 * 4) : m;
+        if (mf < m) RGF_HIP(hipMemsetAsync(d, 0, 2 * S * SE * sizeof(float), s));
         convert<float, double>(s, N, m, P(r, 0), t, sc);
         for (int64_t n = 0; n < S; ++n) {
             const CMat<double> rn = P(r, n + 1);
-            sweep_forward(s, N, m, Fn(n), t, D(n), n + 1 < S ? &rn : nullptr, sc, sweep3m, wk.first, wk.second);
+            sweep_forward(s, N, n + 2 < S ? mf : m, Fn(n), t, D(n), n + 1 < S ? &rn : nullptr, sc, sweep3m,
+                          wk.first, wk.second);
+            if (n + 2 < S) skipped_cols += m - mf;
         }
         // backward sweep: d_n += g_n d_{n+1}
         for (int64_t n = S - 2; n >= 0; --n)
@@ -494,7 +503,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     }
     w.refine_iters = it;
     w.refine_residual = rel;
-    w.flops += (double)it * (2.0 * S - 1) * 8 * Nd * Nd * m * (sweep3m ? 0.75 : 1.0);
+    w.flops += ((double)it * (2.0 * S - 1) * m - (double)skipped_cols) * 8 * Nd * Nd * (sweep3m ? 0.75 : 1.0);
     // S blocks from the boundary slices: C_L psi_0, C_R psi_{L+1}
     const int64_t po = w.outL.size(), qo = w.outR.size();
     if (po) {
