@@ -1863,8 +1863,15 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         Rold = CMat<T>(ws + (size_t)2 * b * N, ws + pl + (size_t)2 * b * N, N);
         D = CMat<T>(ws + (size_t)3 * b * N, ws + pl + (size_t)3 * b * N, b);
     }
-    auto p_of = [&](int64_t q) { return q * b; };
-    auto nb_of = [&](int64_t q) { return q < nblk ? (int)std::min<int64_t>(b, N - q * b) : 0; };
+    // A 64-wide first block when b = 128 (all block edges stay on the 64-wide
+    // tile grid): the first diagonal inverse, which nothing overlaps, costs
+    // about a third of a 128-wide one.
+    const int b0 = (b == 128 && N > 2 * b) ? 64 : b;
+    const int64_t nbl = b0 < b ? 1 + (N - b0 + b - 1) / b : nblk;
+    auto p_of = [&](int64_t q) { return q == 0 ? (int64_t)0 : b0 + (q - 1) * (int64_t)b; };
+    auto nb_of = [&](int64_t q) {
+        return q < nbl ? (int)std::min<int64_t>(q == 0 ? b0 : b, N - p_of(q)) : 0;
+    };
     // A[P, J] = D Rold[:, J] and A[I, P] = -Aold[I, :] D outside P.  The
     // blocks the next copies need from these panels, A[Q, P] and A[P, Q], are
     // also stored to Rold[:, P] and Aold[Q][P, :] (Rold[:, P] is not read
@@ -1917,7 +1924,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         schur(1, 0, evS[0]);
         launch_diag<T>(s, A, p_of(1), nb_of(1), D, dev_status);
     }
-    for (int64_t q = 0; q < nblk; ++q) {
+    for (int64_t q = 0; q < nbl; ++q) {
         const int64_t p0 = p_of(q), p1 = p_of(q + 1);
         const int nb = nb_of(q), nb1 = nb_of(q + 1), nb2 = nb_of(q + 2);
         const int64_t p2 = p1 + nb1, p3 = p2 + nb2;
