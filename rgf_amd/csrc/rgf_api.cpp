@@ -919,6 +919,7 @@ rgf_status rgf_create(rgf_handle* h, int32_t device) {
         RGF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
         RGF_HIP(hipStreamCreateWithFlags(&c->ov.s2, hipStreamNonBlocking));
+        RGF_HIP(hipStreamCreateWithFlags(&c->ov.s3, hipStreamNonBlocking));
         // the events order kernels of one device (a device-scope release is
         // enough); they may complete with a kernel launch (hipExtLaunchKernel)
         for (auto& e : c->ov.ev) RGF_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
@@ -944,6 +945,7 @@ rgf_status rgf_destroy(rgf_handle h) {
         for (auto& e : h->ov.ev)
             if (e) RGF_HIP(hipEventDestroy(e));
         if (h->ov.s2) RGF_HIP(hipStreamDestroy(h->ov.s2));
+        if (h->ov.s3) RGF_HIP(hipStreamDestroy(h->ov.s3));
     });
     delete h;
     return st;

@@ -68,10 +68,11 @@ struct KernelTimer {
     ~KernelTimer();
 };
 
-// Second stream and events for the look-ahead of the slice inverse.
+// Second and third streams and the events of the look-ahead of the slice
+// inverse.
 struct Overlap {
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipStream_t s2 = nullptr, s3 = nullptr;
+    hipEvent_t ev[8] = {};
 };
 
 // ---- kernel launchers (rgf_kernels.hip) -------------------------------------

@@ -1255,9 +1255,6 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                                                                    T* Dre, T* Dim, int64_t ldd, int* status) {
     using V = typename Mfma<T>::v4;
     constexpr int NW = NB / 16;  // wavefronts; wavefront w owns block column w
-    // the next owner overlaps its remaining tile updates with the elimination
-    // (needs the register room of one wavefront per SIMD)
-    constexpr bool INTERLEAVE = NW <= 4;
     constexpr int NT = NW * 64;
     constexpr int PW = 16;       // panel width
     constexpr int LDP = NB + 1;
@@ -1271,19 +1268,24 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
     __shared__ int pivs[NB];
     __shared__ int bad_s, any_swap;
     __shared__ T colr[2][NB], coli[2][NB], rowr[2][PW], rowi[2][PW], pvr[PW], pvi[PW];
-    __shared__ int okflag[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
 
-    // ---- fast path: no row interchanges, the matrix in registers.  Panel p
-    // (block column p) belongs to wavefront p: it publishes the column panel
-    // A[:, P] and D = A_PP^-1 through LDS, then every wavefront w != p forms
-    // R = D A[P, w] and A[I, w] -= A[I, P] R (I = rows outside P), and
-    // wavefront p replaces its column by -A[I, P] D and D.  If a panel fails
-    // the growth checks (max|D| max|A[:, P]|, and the largest intermediate
-    // value of the tile elimination over max|A_PP|, above 31.6 in FP32 or
-    // 100 in FP64) or meets a zero pivot, the block is factored again below with
+    // ---- fast path: no row interchanges, the matrix in registers.  Block
+    // column w belongs to wavefront w.  Per panel p (Q = p + 1): after the
+    // column panel A[:, P] and D_P = A_PP^-1 are published in LDS, every
+    // wavefront w != p forms R_w = D_P A[P, w]; the owner of Q also brings its
+    // pivot tile up to date and puts its column in LDS as the next column
+    // panel (old tiles, R_Q in the rows P, the updated pivot tile).  After a
+    // barrier the owner of Q inverts its pivot tile (D_Q) while every other
+    // wavefront w updates its own tiles, A[I, w] -= A[I, P] R_w, and the tile
+    // (w, Q) of the next panel in LDS, A[w, Q] -= A[w, P] R_Q: the owner's
+    // elimination runs beside all the trailing tile updates.  The owner of P
+    // replaces its column by -A[I, P] D_P and D_P.  If a panel fails the
+    // growth checks (max|D| max|A[:, P]|, and the largest intermediate value
+    // of the tile elimination over max|A_PP|, above 31.6 in FP32 or 100 in
+    // FP64) or meets a zero pivot, the block is factored again below with
     // partial pivoting.
     {
         const int g = lane >> 4, c = lane & 15;
@@ -1318,14 +1320,44 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
         T* const fi = &si[0][0];
         auto cbuf = [&](int buf, int plane) { return (plane ? fi : fr) + buf * NB * CLD; };
         auto dbuf = [&](int buf, int plane) { return (plane ? fi : fr) + 2 * NB * CLD + buf * 16 * CLD; };
-        // D of the panel this wavefront owns (kept from its factorization)
+        // per panel: elimination verdict, max|D|^2, and max|x|^2 of each tile of
+        // its column panel (written by the wavefront that forms the tile); the
+        // growth checks are evaluated once, after the last panel
+        __shared__ int okp[NBLK];
+        __shared__ float dmax[NBLK], tmax[NBLK][NBLK];
+        auto tile_max = [&](const V& vr, const V& vi) {
+            float m = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) m = fmaxf(m, (float)(vr[r] * vr[r] + vi[r] * vi[r]));
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            return m;
+        };
+        // tile (bi, column) <-> column panel buffer in the accumulator layout
+        auto put_tile = [&](int buf, int bi, const V& vr, const V& vi) {
+            T* const cr = cbuf(buf, 0) + bi * 16 * CLD;
+            T* const ci = cbuf(buf, 1) + bi * 16 * CLD;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cr[Mfma<T>::row(g, r) * CLD + c] = vr[r];
+                ci[Mfma<T>::row(g, r) * CLD + c] = vi[r];
+            }
+        };
+        auto get_tile = [&](int buf, int bi, V& vr, V& vi) {
+            const T* const cr = cbuf(buf, 0) + bi * 16 * CLD;
+            const T* const ci = cbuf(buf, 1) + bi * 16 * CLD;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                vr[r] = cr[Mfma<T>::row(g, r) * CLD + c];
+                vi[r] = ci[Mfma<T>::row(g, r) * CLD + c];
+            }
+        };
+        const float lim = sizeof(T) == 4 ? 1e3f : 1e4f;  // squared growth bounds
+        // D of the panel this wavefront owns (kept from its elimination)
         V dr = V{0, 0, 0, 0}, di = V{0, 0, 0, 0};
-        // Factor panel q (called by wavefront q once its block column holds
-        // the values after panel q-1): invert the pivot tile into dr/di while
-        // `other` (if given) brings the remaining tiles up to date, then
-        // publish the column panel and D for panel q and the verdict of the
-        // growth checks.
-        auto factor = [&](int q, auto&& other) {
+        // Invert the pivot tile of panel q (in xr[q], up to date) into dr/di;
+        // publish D_q, max|D_q|^2 and the verdict of the elimination checks.
+        auto factor = [&](int q) {
 #pragma unroll
             for (int bi = 0; bi < NW; ++bi)
                 if (bi == q) {
@@ -1337,23 +1369,8 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
             for (int r = 0; r < 4; ++r) mt = fmaxf(mt, (float)(dr[r] * dr[r] + di[r] * di[r]));
             bool bad = false;
             float mg = 0.f;
-            static_for<0, 16>([&](auto kc) {
-                gj_step16<T, decltype(kc)::value>(dr, di, g, c, bad, mg);
-                other(decltype(kc)::value);
-            });
+            static_for<0, 16>([&](auto kc) { gj_step16<T, decltype(kc)::value>(dr, di, g, c, bad, mg); });
             const int buf = q & 1;
-            T* const cr = cbuf(buf, 0);
-            T* const ci = cbuf(buf, 1);
-            float ma = 0.f;
-#pragma unroll
-            for (int bi = 0; bi < NW; ++bi)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int i = bi * 16 + Mfma<T>::row(g, r);
-                    cr[i * CLD + c] = xr[bi][r];
-                    ci[i * CLD + c] = xi[bi][r];
-                    ma = fmaxf(ma, (float)(xr[bi][r] * xr[bi][r] + xi[bi][r] * xi[bi][r]));
-                }
             float md = 0.f;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -1361,29 +1378,35 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                 dbuf(buf, 1)[Mfma<T>::row(g, r) * CLD + c] = di[r];
                 md = fmaxf(md, (float)(dr[r] * dr[r] + di[r] * di[r]));
             }
-            const float wa = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(ma)));
             const float wd = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(md)));
             const float wt = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mt)));
             const float wg = __uint_as_float((unsigned)wave_max_u64(__float_as_uint(mg)));
-            // bounds, squared: multipliers max|A[:, P]| max|D| and the growth
-            // of the in-tile elimination max|x| / max|A_PP| stay below 31.6
-            // (FP32) or 100 (FP64)
-            const float lim = sizeof(T) == 4 ? 1e3f : 1e4f;
-            const float gr = wa * wd;
+            // growth of the in-tile elimination max|x| / max|A_QQ|, squared
             const bool grow_ok = isfinite(wg) && wg <= lim * fmaxf(wt, 1e-30f);
-            if (lane == 0) okflag[buf] = (!bad && grow_ok && isfinite(gr) && gr <= lim) ? 1 : 0;
+            if (lane == 0) {
+                okp[q] = (!bad && grow_ok) ? 1 : 0;
+                dmax[q] = wd;
+            }
         };
-        if (wave == 0) factor(0, [](int) {});
-        bool ok = true;
-        for (int p = 0; p < NW; ++p) {
-            const int buf = p & 1;
+        if (wave == 0) {
+            float m = 0.f;
+#pragma unroll
+            for (int bi = 0; bi < NW; ++bi) {
+                put_tile(0, bi, xr[bi], xi[bi]);
+                m = fmaxf(m, tile_max(xr[bi], xi[bi]));
+            }
+            if (lane == 0)
+                for (int bi = 0; bi < NW; ++bi) tmax[0][bi] = m;
+            factor(0);
+        }
+        // A panel that fails the checks only makes the later panels produce
+        // values that are discarded: nothing is written before the verdict.
+        for (int p = 0; p < NBLK; ++p) {
+            const int buf = p & 1, q = p + 1, nbuf = q & 1;
             const T* const cr = cbuf(buf, 0);
             const T* const ci = cbuf(buf, 1);
             __syncthreads();  // panel p's column and D are published
-            if (!okflag[buf]) {
-                ok = false;
-                break;
-            }
+            V rr = V{0, 0, 0, 0}, ri = V{0, 0, 0, 0};
             if (wave == p) {
                 // own column: -A[I, P] D, and D in the pivot rows
 #pragma unroll
@@ -1398,66 +1421,85 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
                         xi[bi] = ni;
                     }
                 }
-                continue;
-            }
-            // R = D A[P, w]; then A[I, w] -= A[I, P] R and A[P, w] = R
-            V br = xr[0], bim = xi[0];
+            } else {
+                // R = D A[P, w], stored in place of A[P, w]
+                V br = xr[0], bim = xi[0];
 #pragma unroll
-            for (int bi = 1; bi < NW; ++bi)
-                if (bi == p) {
-                    br = xr[bi];
-                    bim = xi[bi];
-                }
-            V rr = V{0, 0, 0, 0}, ri = V{0, 0, 0, 0};
-            mma_tile16<T>(rr, ri, dbuf(buf, 0), dbuf(buf, 1), CLD, br, bim, T(1), g, c);
-#pragma unroll
-            for (int bi = 0; bi < NW; ++bi)
-                if (bi == p) {
-                    xr[bi] = rr;
-                    xi[bi] = ri;
-                }
-            if (wave == p + 1 && !INTERLEAVE) {
-                // next panel's owner: bring the column up to date, then factor
+                for (int bi = 1; bi < NW; ++bi)
+                    if (bi == p) {
+                        br = xr[bi];
+                        bim = xi[bi];
+                    }
+                mma_tile16<T>(rr, ri, dbuf(buf, 0), dbuf(buf, 1), CLD, br, bim, T(1), g, c);
 #pragma unroll
                 for (int bi = 0; bi < NW; ++bi)
-                    if (bi != p)
-                        mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
-                                      c);
-                factor(p + 1, [](int) {});
-            } else if (wave == p + 1) {
-                // next panel's owner: its pivot tile first, then the
-                // elimination with the other tiles' MFMA steps in between
-#pragma unroll
-                for (int bi = 0; bi < NW; ++bi)
-                    if (bi == p + 1)
-                        mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
-                                      c);
-                factor(p + 1, [&](int k) {
-                    // MFMA steps spread over the 16 elimination steps: tile
-                    // bi takes steps 16 bi / NW .. 16 (bi + 1) / NW - 1
+                    if (bi == p) {
+                        xr[bi] = rr;
+                        xi[bi] = ri;
+                    }
+                if (wave == q) {
+                    // the next column panel: old tiles, R_Q in the rows P, and
+                    // the pivot tile brought up to date here
 #pragma unroll
                     for (int bi = 0; bi < NW; ++bi) {
-                        constexpr int per = 16 / NW;  // elimination steps per tile
-                        if (k / per != bi || bi == p || bi == p + 1) continue;
-                        constexpr int sper = 4 / per;  // MFMA steps per elimination step
-#pragma unroll
-                        for (int u = 0; u < sper; ++u) {
-                            const int s4 = (k % per) * sper + u;
-                            mma_step16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr[s4],
-                                          ri[s4], T(-1), g, c, s4);
-                        }
+                        if (bi == q)
+                            mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1),
+                                          g, c);
+                        put_tile(nbuf, bi, xr[bi], xi[bi]);
                     }
-                });
-            } else {
+                    float mp = 0.f, mq = 0.f;
+#pragma unroll
+                    for (int bi = 0; bi < NW; ++bi) {
+                        if (bi == p) mp = tile_max(xr[bi], xi[bi]);
+                        if (bi == q) mq = tile_max(xr[bi], xi[bi]);
+                    }
+                    if (lane == 0) {
+                        tmax[q][p] = mp;
+                        tmax[q][q] = mq;
+                    }
+                }
+            }
+            if (q == NBLK) {
+                // last panel: the remaining tile updates, no next panel
+                if (wave != p) {
+#pragma unroll
+                    for (int bi = 0; bi < NW; ++bi)
+                        if (bi != p)
+                            mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1),
+                                          g, c);
+                }
+                break;
+            }
+            __syncthreads();  // the next column panel's old tiles and R_Q are in LDS
+            if (wave == q) {
+                factor(q);
+            } else if (wave != p) {
 #pragma unroll
                 for (int bi = 0; bi < NW; ++bi)
                     if (bi != p)
                         mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
                                       c);
+                // tile (w, Q) of the next panel: A[w, Q] -= A[w, P] R_Q
+                V tr, ti, qr, qi;
+                get_tile(nbuf, wave, tr, ti);
+                get_tile(nbuf, p, qr, qi);
+                mma_tile16<T>(tr, ti, cr + wave * 16 * CLD, ci + wave * 16 * CLD, CLD, qr, qi, T(-1), g, c);
+                put_tile(nbuf, wave, tr, ti);
+                const float m = tile_max(tr, ti);
+                if (lane == 0) tmax[q][wave] = m;
             }
         }
+        __syncthreads();  // every verdict is in LDS; the panel buffers (aliasing sr / si) are free
+        // checks, squared: multipliers max|A[:, P]| max|D_P| and the growth of
+        // the in-tile elimination stay below 31.6 (FP32) or 100 (FP64)
+        bool ok = true;
+        for (int p = 0; p < NBLK; ++p) {
+            float m = 0.f;
+            for (int bi = 0; bi < NBLK; ++bi) m = fmaxf(m, tmax[p][bi]);
+            const float gr = m * dmax[p];
+            ok = ok && okp[p] && isfinite(gr) && gr <= lim;
+        }
         if (ok) {
-            __syncthreads();  // the last panel's LDS buffers alias sr / si
 #pragma unroll
             for (int bi = 0; bi < NW; ++bi)
 #pragma unroll
@@ -1827,7 +1869,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         RGF_HIP(hipGetLastError());
         return;
     }
-    if (!ov || !ov->s2) {
+    if (!ov || !ov->s2 || !ov->s3) {
         for (int64_t p0 = 0; p0 < N; p0 += b) {
             const int nb = (int)std::min<int64_t>(b, N - p0);
             panels(s, p0, nb, 0);
@@ -1852,8 +1894,9 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
     //                 step q (its single work group needs a whole CU).
     // Every write to A is in place and no write-back follows the bulk update;
     // the second stream waits only for the panels of the step it applies.
-    hipStream_t s2 = ov->s2;
+    hipStream_t s2 = ov->s2, s3 = ov->s3;
     hipEvent_t evS[2] = {ov->ev[0], ov->ev[1]}, evT[2] = {ov->ev[2], ov->ev[3]};
+    hipEvent_t evB[2] = {ov->ev[5], ov->ev[6]};
     CMat<T> Aold[2], Rold;
     {
         // workspace: Aold[0], Aold[1] (N x b), Rold (b x N), then D
@@ -1931,6 +1974,10 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         CMat<T> An = Aold[(q + 1) & 1];
         RGF_HIP(hipStreamWaitEvent(s2, evS[q & 1], 0));
         if (nb1 > 0) {
+            // third stream, beside the bulk update of step q: after the bulk
+            // update of step q-1 (rows and columns Q) and P's panels
+            if (q > 0) RGF_HIP(hipStreamWaitEvent(s3, evB[(q - 1) & 1], 0));
+            RGF_HIP(hipStreamWaitEvent(s3, evS[q & 1], 0));
             // Rold = A[Q, J] - Aold[Q, :] A[P, J]; An[I, :] = A[I, Q] - Aold[I, :] A[P, Q]
             const GemmArgs<T> ga = from_source(gemm_args<T>(nb1, N, nb, Aold[q & 1].offset(p1, 0), A.offset(p0, 0),
                                                             Rold, cd(-1, 0), 1, Skip{0, 0, p0, p2}),
@@ -1941,9 +1988,9 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
             if (nb2 > 0) {
                 const GemmArgs<T> gc = gemm_args<T>(nb2, nb2, nb, Aold[q & 1].offset(p2, 0), A.offset(p0, p2),
                                                     A.offset(p2, p2), cd(-1, 0), 1, Skip{});
-                cgemm_triple<T>(s2, ga, gb, gc, use3m, evT[q & 1]);
+                cgemm_triple<T>(s3, ga, gb, gc, use3m, evT[q & 1]);
             } else {
-                cgemm_pair<T>(s2, ga, gb, use3m, evT[q & 1]);
+                cgemm_pair<T>(s3, ga, gb, use3m, evT[q & 1]);
             }
         }
         if (timer) timer->begin(s2);
@@ -1953,6 +2000,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
             const double nn = (double)(N - nb - nb1);
             timer->end(s2, fl * (nn * nn - (double)nb2 * nb2) * nb);
         }
+        RGF_HIP(hipEventRecord(evB[q & 1], s2));
         if (nb1 > 0) {
             RGF_HIP(hipStreamWaitEvent(s, evT[q & 1], 0));
             if (nb2 > 0) {
