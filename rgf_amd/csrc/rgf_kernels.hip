@@ -519,8 +519,54 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     // operand column feeding MFMA row i is bperm(i) (FP64 accumulators hold
     // rows g4 + 4r, which the permutation sends to columns 4 g4 + r).
     auto bperm = [](int i) { return (TR && sizeof(T) == 8) ? 4 * (i & 3) + (i >> 2) : i; };
+    // TR with four products: the beta = 1 source is read at the start of the
+    // last K chunk, into cx (its load latency hidden behind that chunk's
+    // MFMAs), and added in the epilogue; the accumulators start from zero.
+    // TR with 3M (more accumulators): it is read first, into the accumulators.
+    constexpr bool LATE = TR && !M3;
+    V cxr[LATE ? FM : 1][LATE ? FN : 1], cxi[LATE ? FM : 1][LATE ? FN : 1];
+    // (the lambda reads the source through locals: kernel arguments taken
+    // by reference inside a lambda end up in scratch memory)
+    const T* const xsr = g.Xre;
+    const T* const xsi = g.Xim;
+    const int64_t ldxs = g.ldx;
+    const int beta = g.beta;
+    auto load_c = [&]() {
+        const bool vx = (ldxs % 4) == 0 && (((uintptr_t)xsr | (uintptr_t)xsi) & 15) == 0;
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b) {
+                cxr[a][b] = V{0, 0, 0, 0};
+                cxi[a][b] = V{0, 0, 0, 0};
+                const int gi = tile_m + wm * TM + a * 16 + l16;
+                const int gj = tile_n + wn * TN + b * 16 + 4 * g4;
+                if (gi < M) {
+                    const int64_t o = (int64_t)gi * ldxs + gj;
+                    if (vx && gj + 4 <= N) {
+#pragma unroll
+                        for (int u = 0; u < 4; u += VW) {
+                            const VT xr = *(const VT*)(xsr + o + u);
+                            const VT xi = *(const VT*)(xsi + o + u);
+#pragma unroll
+                            for (int v = 0; v < VW; ++v) {
+                                cxr[a][b][u + v] = xr[v];
+                                cxi[a][b][u + v] = xi[v];
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (gj + r < N) {
+                                cxr[a][b][r] = xsr[o + r];
+                                cxi[a][b][r] = xsi[o + r];
+                            }
+                    }
+                }
+            }
+    };
     if (g.beta) {
-        if constexpr (TR) {
+        if constexpr (TR && !LATE) {
             const bool vx = (g.ldx % 4) == 0 && (((uintptr_t)g.Xre | (uintptr_t)g.Xim) & 15) == 0;
 #pragma unroll
             for (int a = 0; a < FM; ++a)
@@ -552,7 +598,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                         if constexpr (M3) ri[a][b] += rr[a][b];
                     }
                 }
-        } else {
+        } else if constexpr (!TR) {
 #pragma unroll
             for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -623,9 +669,8 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     fetch(0);
     stash(0);
     __syncthreads();
-    for (int kc = 0; kc < nk; ++kc) {
-        const int buf = kc & 1;
-        if (kc + 1 < nk) fetch((kc + 1) * BK);
+    // MFMA work of the K chunk in LDS buffer buf
+    auto chunk = [&](int buf) {
 #pragma unroll
         for (int kb = 0; kb < BK; kb += 16) {
             T ar[FM][4], ai[FM][4], br[FN][4], bi[FN][4];
@@ -703,8 +748,28 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     for (int b = 0; b < FN; ++b) ri[a][b] = mm(ai[a][s2], br[b][s2], ri[a][b]);
             }
         }
-        if (kc + 1 < nk) stash(buf ^ 1);
-        __syncthreads();
+    };
+    if constexpr (LATE) {
+        // the last chunk peeled off: no prefetch there, the C reads instead
+        for (int kc = 0; kc + 1 < nk; ++kc) {
+            const int buf = kc & 1;
+            fetch((kc + 1) * BK);
+            __builtin_amdgcn_sched_barrier(0);  // keep the next chunk's loads ahead of the MFMAs
+            chunk(buf);
+            __builtin_amdgcn_sched_barrier(0);  // and their use after them
+            stash(buf ^ 1);
+            __syncthreads();
+        }
+        if (beta) load_c();
+        chunk((nk - 1) & 1);
+    } else {
+        for (int kc = 0; kc < nk; ++kc) {
+            const int buf = kc & 1;
+            if (kc + 1 < nk) fetch((kc + 1) * BK);
+            chunk(buf);
+            if (kc + 1 < nk) stash(buf ^ 1);
+            __syncthreads();
+        }
     }
 
     // epilogue (the accumulators started from C when beta = 1)
@@ -725,6 +790,10 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     if constexpr (A3) vr[r] -= ii[a][b][r];
                     vi[r] = ri[a][b][r];
                     if constexpr (M3) vi[r] -= rr[a][b][r] + ii[a][b][r];
+                    if (LATE && beta) {
+                        vr[r] += cxr[a][b][r];
+                        vi[r] += cxi[a][b][r];
+                    }
                 }
                 const int64_t o = (int64_t)gi * ldc + gj;
                 const bool whole = vc && gj + 4 <= N && !(gj + 4 > g.sclo && gj < g.schi) &&
