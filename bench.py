@@ -19,8 +19,8 @@ traffic (HBM bytes per launch) and MFMA utilisation of that kernel come from
 the rocprofv3 counter summary profiles/pmc_slice_inverse_n4096.md (FETCH_SIZE
 and WRITE_SIZE scaled by a copy kernel of known size, MfmaUtil).
 The mixed path refines to a relative residual of --refine-tol, by default the
-metric's own S tolerance 1e-9 (three correction sweeps here: residual 9.6e-13,
-S error against the FP64 path 4.0e-13); the line reports the measured S error
+metric's own S tolerance 1e-9 (three correction sweeps here: residual 4.3e-14,
+S error against the FP64 path 5e-15); the line reports the measured S error
 and whether it is inside the bound ("s_err_within_bound").
 cpu_baseline: the FP64 oracle (numpy/LAPACK) general-scheme slice update at
 the same N, timed on the host cores for a bounded number of slices.

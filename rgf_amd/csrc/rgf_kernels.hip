@@ -1310,6 +1310,18 @@ __device__ __forceinline__ void mma_tile16(typename Mfma<T>::v4& accr, typename 
     for (int s = 0; s < 4; ++s) mma_step16<T>(accr, acci, Ar, Ai, ld, br[s], bi[s], sgn, g, c, s);
 }
 
+// acc -= A * B for 16x16 complex tiles, the product summed from zero and
+// added once (one rounding at the scale of acc instead of one per MFMA step)
+template <typename T>
+__device__ __forceinline__ void mma_tile16_sub(typename Mfma<T>::v4& accr, typename Mfma<T>::v4& acci, const T* Ar,
+                                               const T* Ai, int ld, const typename Mfma<T>::v4& br,
+                                               const typename Mfma<T>::v4& bi, int g, int c) {
+    typename Mfma<T>::v4 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
+    mma_tile16<T>(pr, pi, Ar, Ai, ld, br, bi, T(1), g, c);
+    accr -= pr;
+    acci -= pi;
+}
+
 // Compile-time loop helper: f(std::integral_constant<int, K>) for K in [0, N).
 template <int K, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -1512,8 +1524,7 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
 #pragma unroll
                     for (int bi = 0; bi < NW; ++bi) {
                         if (bi == q)
-                            mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1),
-                                          g, c);
+                            mma_tile16_sub<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, g, c);
                         put_tile(nbuf, bi, xr[bi], xi[bi]);
                     }
                     float mp = 0.f, mq = 0.f;
@@ -1534,8 +1545,7 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
 #pragma unroll
                     for (int bi = 0; bi < NW; ++bi)
                         if (bi != p)
-                            mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1),
-                                          g, c);
+                            mma_tile16_sub<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, g, c);
                 }
                 break;
             }
@@ -1546,13 +1556,12 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
 #pragma unroll
                 for (int bi = 0; bi < NW; ++bi)
                     if (bi != p)
-                        mma_tile16<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, T(-1), g,
-                                      c);
+                        mma_tile16_sub<T>(xr[bi], xi[bi], cr + bi * 16 * CLD, ci + bi * 16 * CLD, CLD, rr, ri, g, c);
                 // tile (w, Q) of the next panel: A[w, Q] -= A[w, P] R_Q
                 V tr, ti, qr, qi;
                 get_tile(nbuf, wave, tr, ti);
                 get_tile(nbuf, p, qr, qi);
-                mma_tile16<T>(tr, ti, cr + wave * 16 * CLD, ci + wave * 16 * CLD, CLD, qr, qi, T(-1), g, c);
+                mma_tile16_sub<T>(tr, ti, cr + wave * 16 * CLD, ci + wave * 16 * CLD, CLD, qr, qi, g, c);
                 put_tile(nbuf, wave, tr, ti);
                 const float m = tile_max(tr, ti);
                 if (lane == 0) tmax[q][wave] = m;
