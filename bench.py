@@ -18,10 +18,13 @@ of them would add ~10 us of event latency per block step).
 traffic (HBM bytes per launch) and MFMA utilisation of that kernel come from
 the rocprofv3 counter summary profiles/pmc_slice_inverse_n4096.md (FETCH_SIZE
 and WRITE_SIZE scaled by a copy kernel of known size, MfmaUtil).
-The mixed path refines to a relative residual of --refine-tol, by default the
-metric's own S tolerance 1e-9 (three correction sweeps here: residual 4.3e-14,
-S error against the FP64 path 5e-15); the line reports the measured S error
-and whether it is inside the bound ("s_err_within_bound").
+The mixed path stops refining once the estimated max error of the S entries
+is below --refine-s-tol (the metric's bound, 1e-9: q/(1-q) times the change
+of S over the last sweep, q the ratio of the last two residuals, with a
+factor 2 to spare) or the relative residual is below --refine-tol (also
+1e-9).  Two correction sweeps here (residual 1.9e-9, S error against the
+FP64 path 1.8e-10); the line reports the estimate, the measured S error and
+whether it is inside the bound ("s_err_within_bound").
 cpu_baseline: the FP64 oracle (numpy/LAPACK) general-scheme slice update at
 the same N, timed on the host cores for a bounded number of slices.
 """
@@ -137,6 +140,8 @@ def main():
                     help="no look-ahead stream in the slice inverse (for hardware-counter runs)")
     ap.add_argument("--refine-tol", type=float, default=1e-9,
                     help="mixed path: stop the FP64 refinement at this relative residual")
+    ap.add_argument("--refine-s-tol", type=float, default=S_ERR_BOUND,
+                    help="mixed path: stop once the estimated S error is below this (0 = off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -158,6 +163,7 @@ def main():
         o = dict(opts, profile=profile)
         if prec == "fp32_refined":
             o["refine_tol"] = args.refine_tol
+            o["refine_s_tol"] = args.refine_s_tol
         return rgf_amd.cal_smatrix_RGF_3D(syst, out, inn, precision=prec, device=0, options=o, return_info=True)
 
     print(f"# workload: 3D N={N}, {L} slices, full S; warmup {args.warmup}", file=sys.stderr, flush=True)
@@ -188,7 +194,8 @@ def main():
                    "n_out": int(S32.shape[0]), "n_in": int(S32.shape[1]), "parallelism": "single GPU"},
         "roofline": roofline(agg, "fp32"),
         "effective_tflops_reference_formulation": round(value * 32.0 * float(N) ** 3 / 1e12, 1),
-        "refine_tol": args.refine_tol,
+        "refine_tol": args.refine_tol, "refine_s_tol": args.refine_s_tol,
+        "refine_s_err_estimate": infos[-1].get("refine_s_err_estimate"),
         "complex_3m": int(rgf_amd.api._options("fp32_refined", complex_3m=args.complex_3m).complex_3m),
         "refine_iters": infos[-1]["refine_iters"], "refine_residual": infos[-1]["refine_residual"],
         "unitarity_defect_fp32_refined": unitarity(S32),

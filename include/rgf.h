@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define RGF_ABI_VERSION 1
+#define RGF_ABI_VERSION 2  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate */
 #define RGF_ALL (-1) /* selection count meaning "all propagating channels 1..N_prop" */
 
 typedef enum {
@@ -104,6 +104,9 @@ typedef struct {
     int32_t batch_concurrency;/* rgf_solve_batched: solves in flight at once, each on
                                  its own streams and workspace; 0 = 8                 */
     int32_t reserved[3];
+    double refine_s_tol;      /* FP32_REFINED, > 0: also stop once the estimated max
+                                 error of the S entries is below this (see
+                                 rgf_info.refine_s_err_estimate); 0 = off (default)  */
 } rgf_options;
 
 typedef struct {
@@ -122,6 +125,10 @@ typedef struct {
     double kernel_seconds;
     double kernel_flops;
     int64_t kernel_launches;
+    /* with opt.refine_s_tol: estimated max error of the S entries at the
+       stop, 4 q/(1-q) max|S_k - S_(k-1)| with q the ratio of the last two
+       residuals (linear convergence, a factor 2 to spare); 0 otherwise */
+    double refine_s_err_estimate;
 } rgf_info;
 
 /* Lead channels (setup_channels.m:30-82; setup_channels_3D.m:335-370).

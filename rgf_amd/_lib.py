@@ -52,6 +52,7 @@ class Options(C.Structure):
         ("precision", C.c_int32), ("max_refine_iter", C.c_int32), ("refine_tol", C.c_double),
         ("block_size", C.c_int32), ("complex_3m", C.c_int32), ("profile", C.c_int32),
         ("single_stream", C.c_int32), ("batch_concurrency", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("refine_s_tol", C.c_double),
     ]
 
 
@@ -63,6 +64,7 @@ class Info(C.Structure):
         ("refine_residual", C.c_double), ("seconds_total", C.c_double), ("seconds_setup", C.c_double),
         ("seconds_slices", C.c_double), ("seconds_refine", C.c_double), ("flops_slices", C.c_double),
         ("kernel_seconds", C.c_double), ("kernel_flops", C.c_double), ("kernel_launches", C.c_int64),
+        ("refine_s_err_estimate", C.c_double),
     ]
 
     def as_dict(self) -> dict:

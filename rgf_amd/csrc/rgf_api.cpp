@@ -168,6 +168,7 @@ struct Work {
     KernelTimer* timer = nullptr;
     int refine_iters = 0;
     double refine_residual = 0;
+    double refine_s_err = 0;  // estimated S error at the stop (refine_s_tol)
     double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
 };
 
@@ -449,6 +450,25 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     double rel = 1.0;
     int it = 0;
     int64_t skipped_cols = 0;  // sweep product columns left out in the first correction
+    // optional stop on the S error: the boundary projections C psi of the
+    // last two iterates and the ratio q of their residuals give the error of
+    // the latest as q/(1-q) times their difference (linear convergence)
+    const int64_t npo = (int64_t)w.outL.size(), nqo = (int64_t)w.outR.size();
+    const bool s_stop = o.refine_s_tol > 0 && npo + nqo > 0;
+    CMat<double> Tcur, Tprev;
+    if (s_stop) {
+        Tcur = dmat<double>(c, "Tcur", npo + nqo, m);
+        Tprev = dmat<double>(c, "Tprev", npo + nqo, m);
+    }
+    auto project = [&](CMat<double> T) {
+        if (npo) cgemm<double>(s, npo, m, N, w.CL, P(psi, 0), T, cd(1, 0), 0, 0, 0, 0, wk64.first, wk64.second);
+        if (nqo)
+            cgemm<double>(s, nqo, m, N, w.CR, P(psi, L + 1), T.offset(npo, 0), cd(1, 0), 0, 0, 0, 0, wk64.first,
+                          wk64.second);
+    };
+    double rel_prev = 1.0;
+    w.refine_s_err = 0;
+    bool s_done = false;
     for (;; ++it) {
         double rmax;
         if (it == 0) {
@@ -479,7 +499,22 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         }
         rel = rmax / bmax;
         if (!std::isfinite(rel)) throw Error(RGF_ERR_NUMERICAL, "non-finite residual in refinement");
-        if (rel <= tol || it >= maxit) break;
+        if (s_stop && it >= 1) {
+            project(Tcur);
+            if (it >= 2) {
+                // Tprev <- Tprev - Tcur; S = -2i T - delta, so |dS| = 2 |dT|
+                accumulate<double>(s, npo + nqo, m, Tcur, Tprev, -1.0);
+                const double dS = 2.0 * max_abs(s, (npo + nqo) * m, Tprev.re, Tprev.im, c->d_scratch);
+                const double q = rel / rel_prev;
+                if (q < 0.5) {
+                    w.refine_s_err = 2.0 * dS * q / (1.0 - q);
+                    s_done = w.refine_s_err <= o.refine_s_tol;
+                }
+            }
+            copy_mat<double>(s, npo + nqo, m, Tcur, Tprev);
+        }
+        rel_prev = rel;
+        if (rel <= tol || s_done || it >= maxit) break;
         const double sc = 1.0 / rmax;
         // forward sweep: d_n = g_n t_n, t_0 = r_0, t_{n+1} = r_{n+1} + d_n
         // (scaled by sc; each product's reduction also forms the next t).  In
@@ -550,7 +585,11 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
         release(c, "field_pack");
     }
     w.t_refine = now() - t1;
-    if (rel > tol) {
+    if (s_stop) {
+        release(c, "Tcur");
+        release(c, "Tprev");
+    }
+    if (rel > tol && !s_done) {
         char buf[160];
         snprintf(buf, sizeof buf, "mixed-precision refinement stopped at relative residual %.3e (tolerance %.1e)", rel,
                  tol);
@@ -820,6 +859,7 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
         info->swept_right_to_left = w.mirrored;
         info->refine_iters = w.refine_iters;
         info->refine_residual = w.refine_residual;
+        info->refine_s_err_estimate = w.refine_s_err;
         info->seconds_setup = w.t_setup;
         info->seconds_slices = w.t_slices;
         info->seconds_refine = w.t_refine;

@@ -9,7 +9,7 @@
  * (epsilon, epsilon_L, epsilon_R, k0dx and yBC (2D) or xyBC (3D)); `out`
  * and `in` are cells {'L','R'} or structs with ind_out_L/ind_out_R and
  * ind_in_L/ind_in_R.  A 3D epsilon (or a syst with xyBC) selects the 3D
- * solver.  `opts` may hold precision ('fp64' | 'fp32_refined'), refine_tol,
+ * solver.  `opts` may hold precision ('fp64' | 'fp32_refined'), refine_tol, refine_s_tol,
  * max_refine_iter and checkpoint_every.  The `channels` output of the
  * reference is produced by the thin wrappers cal_smatrix_RGF.m /
  * cal_smatrix_RGF_3D.m next to this file, which call setup_channels.
@@ -237,6 +237,7 @@ static void get_opts(const mxArray* s, rgf_options* o, int32_t* every) {
         o->precision = streq_ci(b, "fp64") ? RGF_PRECISION_FP64 : RGF_PRECISION_FP32_REFINED;
     }
     if ((f = mxGetField(s, 0, "refine_tol"))) o->refine_tol = mxGetScalar(f);
+    if ((f = mxGetField(s, 0, "refine_s_tol"))) o->refine_s_tol = mxGetScalar(f);
     if ((f = mxGetField(s, 0, "max_refine_iter"))) o->max_refine_iter = (int32_t)mxGetScalar(f);
     if ((f = mxGetField(s, 0, "checkpoint_every"))) *every = (int32_t)mxGetScalar(f);
 }

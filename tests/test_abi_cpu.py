@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
     assert declared == set(L.SIGNATURES), declared ^ set(L.SIGNATURES)
-    assert lib.rgf_abi_version() == 1
+    assert lib.rgf_abi_version() == 2
 
 
 def test_status_strings_and_default_options():
@@ -34,6 +34,7 @@ def test_status_strings_and_default_options():
     o_ = L.Options()
     lib.rgf_default_options(C.byref(o_))
     assert o_.precision == 0 and o_.refine_tol == 1e-13 and o_.max_refine_iter == 6
+    assert o_.refine_s_tol == 0 and o_.complex_3m == 2
 
 
 BCS = ["periodic", "Dirichlet", "Neumann", "DirichletNeumann", "NeumannDirichlet", "PEC", "pmc", 0.9, -2.5]

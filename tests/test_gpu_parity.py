@@ -391,5 +391,32 @@ def test_cfg3_full_size_mixed_vs_fp64():
     assert ib["refine_residual"] <= 1e-9 and ib["refine_iters"] <= info["refine_iters"]
     for S in (S64, S32, Sb):
         assert np.abs(S.conj().T @ S - np.eye(182)).max() < TOL_MIXED
-    print(f"cfg3: refine {info['refine_iters']} / {ib['refine_iters']} sweeps, "
-          f"S err {np.abs(S32 - S64).max():.2e} / {np.abs(Sb - S64).max():.2e}", flush=True)
+    Ss, _, is_ = rgf_amd.cal_smatrix_RGF_3D(syst, out, inn, precision="fp32_refined",
+                                            options={"refine_tol": 1e-16, "refine_s_tol": 1e-9}, return_info=True)
+    err_s = np.abs(Ss - S64).max()
+    assert err_s <= is_["refine_s_err_estimate"] <= 1e-9
+    print(f"cfg3: refine {info['refine_iters']} / {ib['refine_iters']} / {is_['refine_iters']} sweeps, "
+          f"S err {np.abs(S32 - S64).max():.2e} / {np.abs(Sb - S64).max():.2e} / {err_s:.2e} "
+          f"(estimate {is_['refine_s_err_estimate']:.2e})", flush=True)
+
+
+def test_refinement_stops_on_s_error_estimate():
+    """refine_s_tol: the refinement stops once q/(1-q) max|S_k - S_(k-1)| (x2)
+    is below the tolerance; the S error against the FP64 path stays below the
+    estimate (2D slabs of several kinds, the residual criterion switched off)."""
+    rng = np.random.default_rng(31)
+    cases = [dict(epsilon=1 + 3 * rng.random((64, 120)), epsilon_L=1.0, epsilon_R=1.0, k0dx=2 * np.pi / 15,
+                  yBC="periodic"),
+             dict(epsilon=1 + 1.25 * (rng.random((96, 80)) < 0.3) + 0.01j, epsilon_L=1.0, epsilon_R=2.25,
+                  k0dx=2 * np.pi / 10, yBC="Dirichlet"),
+             dict(epsilon=1 + rng.random((48, 200)), epsilon_L=1.3, epsilon_R=1.0, k0dx=0.9, yBC=1.1)]
+    for syst in cases:
+        S64, _ = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"])
+        for stol in (1e-6, 1e-9):
+            S, _, info = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"], precision="fp32_refined",
+                                                 options={"refine_s_tol": stol, "refine_tol": 1e-16},
+                                                 return_info=True)
+            err = np.abs(S - S64).max()
+            est = info["refine_s_err_estimate"]
+            assert 0 < est <= stol, (est, stol)
+            assert err <= est + 1e-13, (err, est)
