@@ -217,8 +217,11 @@ rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem* p,
 
 /* Kernel-level entry points (used by the tests and the benchmark).
  * rgf_dense_inverse: in-place inverse of an n x n complex matrix
- * (interleaved, column-major) with the slice-inverse kernel of `precision`.
+ * (interleaved, column-major) with the slice-inverse kernel of `precision`;
+ * RGF_PRECISION_FP32_REFINED | RGF_DENSE_3M selects the FP32 kernels with
+ * three real products per complex product (complex_3m bit 0).
  * rgf_dense_gemm: C = A B, (m x k)(k x n), interleaved column-major. */
+#define RGF_DENSE_3M 0x100
 rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double* A, int32_t block_size);
 rgf_status rgf_dense_gemm(rgf_handle h, int32_t precision, int64_t m, int64_t n, int64_t k,
                           const double* A, const double* B, double* C);

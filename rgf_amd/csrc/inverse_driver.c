@@ -4,6 +4,7 @@
  * (no Python), used as the target of hardware-counter runs.
  *
  * usage: rgf_inverse_driver [n=4096] [precision=1 (0 FP64, 1 FP32 kernels)] [reps=1]
+ *        [3m=0 (1: FP32 kernels with three real products per complex product)]
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -14,6 +15,7 @@ int main(int argc, char** argv) {
     const int64_t n = argc > 1 ? atoll(argv[1]) : 4096;
     const int prec = argc > 2 ? atoi(argv[2]) : RGF_PRECISION_FP32_REFINED;
     const int reps = argc > 3 ? atoi(argv[3]) : 1;
+    const int use3m = argc > 4 ? atoi(argv[4]) : 0;
     if (n <= 0 || reps <= 0) {
         fprintf(stderr, "usage: %s [n] [precision] [reps]\n", argv[0]);
         return 1;
@@ -32,7 +34,7 @@ int main(int argc, char** argv) {
             A[2 * e + 1] = 0.05 * ((rand() % 2001) / 1000.0 - 1.0);
         }
         for (int64_t i = 0; i < n; ++i) A[2 * (i * n + i)] += 6.0;
-        const rgf_status st = rgf_dense_inverse(h, prec, n, A, 0);
+        const rgf_status st = rgf_dense_inverse(h, use3m ? (prec | RGF_DENSE_3M) : prec, n, A, 0);
         printf("rep %d: %s\n", r, st == RGF_OK ? "ok" : rgf_last_error(h));
         fflush(stdout);
         if (st != RGF_OK) return 2;

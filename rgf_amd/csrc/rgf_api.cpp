@@ -1163,6 +1163,10 @@ rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem* p, const r
 rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double* A, int32_t block_size) {
     return guard(h, [&] {
         if (!h || !A || n <= 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "bad dense inverse arguments");
+        const int use3m = (precision & RGF_DENSE_3M) ? 1 : 0;
+        precision &= ~RGF_DENSE_3M;
+        if (use3m && precision == RGF_PRECISION_FP64)
+            throw Error(RGF_ERR_INVALID_ARGUMENT, "RGF_DENSE_3M applies to the FP32 kernels");
         RGF_HIP(hipSetDevice(h->device));
         CMat<double> M = dmat<double>(h, "dense_A", n, n);
         upload_colmajor(h, A, n, n, M);
@@ -1176,7 +1180,7 @@ rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double*
             CMat<float> M32 = dmat<float>(h, "dense_A32", n, n);
             convert<float, double>(h->stream, n, n, M, M32, 1.0);
             float* ws = dbuf<float>(h, "inv_ws32", inverse_workspace_elems<float>(n, b));
-            invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status, 0, nullptr, &h->ov);
+            invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status, use3m, nullptr, &h->ov);
             convert<double, float>(h->stream, n, n, M32, M, 1.0);
         }
         check_status(h);

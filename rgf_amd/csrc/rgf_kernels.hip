@@ -519,11 +519,11 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     // operand column feeding MFMA row i is bperm(i) (FP64 accumulators hold
     // rows g4 + 4r, which the permutation sends to columns 4 g4 + r).
     auto bperm = [](int i) { return (TR && sizeof(T) == 8) ? 4 * (i & 3) + (i >> 2) : i; };
-    // TR with four products: the beta = 1 source is read at the start of the
-    // last K chunk, into cx (its load latency hidden behind that chunk's
-    // MFMAs), and added in the epilogue; the accumulators start from zero.
-    // TR with 3M (more accumulators): it is read first, into the accumulators.
-    constexpr bool LATE = TR && !M3;
+    // TR: the beta = 1 source is read at the start of the last K chunk, into
+    // cx (its load latency hidden behind that chunk's MFMAs), and added in the
+    // epilogue; the accumulators start from zero, so the entries of C are
+    // rounded once per launch instead of once per K step.
+    constexpr bool LATE = TR;
     V cxr[LATE ? FM : 1][LATE ? FN : 1], cxi[LATE ? FM : 1][LATE ? FN : 1];
     // (the lambda reads the source through locals: kernel arguments taken
     // by reference inside a lambda end up in scratch memory)
@@ -566,39 +566,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
             }
     };
     if (g.beta) {
-        if constexpr (TR && !LATE) {
-            const bool vx = (g.ldx % 4) == 0 && (((uintptr_t)g.Xre | (uintptr_t)g.Xim) & 15) == 0;
-#pragma unroll
-            for (int a = 0; a < FM; ++a)
-#pragma unroll
-                for (int b = 0; b < FN; ++b) {
-                    const int gi = tile_m + wm * TM + a * 16 + l16;
-                    const int gj = tile_n + wn * TN + b * 16 + 4 * g4;
-                    if (gi < M) {
-                        const int64_t o = (int64_t)gi * g.ldx + gj;
-                        if (vx && gj + 4 <= N) {
-#pragma unroll
-                            for (int u = 0; u < 4; u += VW) {
-                                const VT xr = *(const VT*)(g.Xre + o + u);
-                                const VT xi = *(const VT*)(g.Xim + o + u);
-#pragma unroll
-                                for (int v = 0; v < VW; ++v) {
-                                    rr[a][b][u + v] = xr[v];
-                                    ri[a][b][u + v] = xi[v];
-                                }
-                            }
-                        } else {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r)
-                                if (gj + r < N) {
-                                    rr[a][b][r] = g.Xre[o + r];
-                                    ri[a][b][r] = g.Xim[o + r];
-                                }
-                        }
-                        if constexpr (M3) ri[a][b] += rr[a][b];
-                    }
-                }
-        } else if constexpr (!TR) {
+        if constexpr (!TR) {
 #pragma unroll
             for (int a = 0; a < FM; ++a)
 #pragma unroll

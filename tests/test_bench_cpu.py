@@ -21,7 +21,7 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_counter_summary_parses():
-    for prec in ("fp32", "fp64"):
+    for prec in ("fp32", "fp32_3m", "fp64"):
         c = bench.counters(prec)
         assert c is not None, prec
         assert c["traffic_bytes"] > c["algorithmic_bytes"] > 0
