@@ -137,7 +137,7 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=2)
     ap.add_argument("--complex-3m", type=int, default=3,
                     help="3-multiplication complex GEMMs in FP32: bit 0 slice inverses, bit 1 refinement sweeps "
-                         "(default 3: both; the library's own default is 2)")
+                         "(default 3: both, as the library's default)")
     ap.add_argument("--block", type=int, default=0, help="slice-inverse panel width (0 = default)")
     ap.add_argument("--single-stream", action="store_true",
                     help="no look-ahead stream in the slice inverse (for hardware-counter runs)")

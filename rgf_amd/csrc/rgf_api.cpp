@@ -942,7 +942,7 @@ void rgf_default_options(rgf_options* o) {
     o->max_refine_iter = 6;
     o->refine_tol = 1e-13;
     o->block_size = 0;
-    o->complex_3m = 2;  // three real products in the refinement sweeps only
+    o->complex_3m = 3;  // three real products in the slice inverses and the refinement sweeps
 }
 
 rgf_status rgf_create(rgf_handle* h, int32_t device) {

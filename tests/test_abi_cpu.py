@@ -34,7 +34,7 @@ def test_status_strings_and_default_options():
     o_ = L.Options()
     lib.rgf_default_options(C.byref(o_))
     assert o_.precision == 0 and o_.refine_tol == 1e-13 and o_.max_refine_iter == 6
-    assert o_.refine_s_tol == 0 and o_.complex_3m == 2
+    assert o_.refine_s_tol == 0 and o_.complex_3m == 3
 
 
 BCS = ["periodic", "Dirichlet", "Neumann", "DirichletNeumann", "NeumannDirichlet", "PEC", "pmc", 0.9, -2.5]
