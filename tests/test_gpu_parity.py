@@ -420,3 +420,20 @@ def test_refinement_stops_on_s_error_estimate():
             est = info["refine_s_err_estimate"]
             assert 0 < est <= stol, (est, stol)
             assert err <= est + 1e-13, (err, est)
+
+
+def test_batched_s_error_stop():
+    """The S-error stop through the batched entry: every S within 1e-9 of the
+    FP64 path with the residual criterion switched off."""
+    rng = np.random.default_rng(41)
+    items = []
+    for k, ny in enumerate([32, 48, 64, 40]):
+        eps = 1 + 2 * rng.random((ny, 60)) + 0.005j * rng.random((ny, 60))
+        syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.0 + 0.3 * (k % 2), k0dx=2 * np.pi / (9 + 2 * k),
+                    yBC=["periodic", "Dirichlet"][k % 2])
+        items.append((syst, ["L", "R"], ["L", "R"]))
+    S64 = rgf_amd.cal_smatrix_RGF_batched(items)
+    S32 = rgf_amd.cal_smatrix_RGF_batched(items, precision="fp32_refined",
+                                          options={"refine_s_tol": 1e-9, "refine_tol": 1e-16})
+    for a, b in zip(S32, S64):
+        assert np.abs(a - b).max() <= 1e-9
