@@ -441,12 +441,18 @@ def build_laplacian_1d(n: int, BC):
     return sp.csr_matrix(M if np.any(M.imag) else M.real)
 
 
-def dense_inverse(A: np.ndarray, precision: str = "fp64", block_size: int = 0, device: int = -1) -> np.ndarray:
-    """In-place-style inverse through the slice-inverse kernel (kernel-level test entry)."""
+RGF_DENSE_3M = 0x100
+
+
+def dense_inverse(A: np.ndarray, precision: str = "fp64", block_size: int = 0, device: int = -1,
+                  complex_3m: bool = False) -> np.ndarray:
+    """In-place-style inverse through the slice-inverse kernel (kernel-level test entry);
+    complex_3m: the FP32 kernels with three real products per complex product."""
     lib = L.load_library()
     M = np.asfortranarray(A, dtype=np.complex128).copy(order="F")
     h = L.handle(device)
-    L.check(lib.rgf_dense_inverse(h, _PRECISIONS[precision], M.shape[0], M.ctypes.data_as(L._dp), block_size), h)
+    prec = _PRECISIONS[precision] | (RGF_DENSE_3M if complex_3m else 0)
+    L.check(lib.rgf_dense_inverse(h, prec, M.shape[0], M.ctypes.data_as(L._dp), block_size), h)
     return M
 
 

@@ -59,6 +59,22 @@ def test_slice_inverse_fp32(n, b):
     assert np.abs(X @ A - np.eye(n)).max() < 5e-5
 
 
+@pytest.mark.parametrize("n,b", [(128, 0), (520, 128), (1000, 128), (301, 64)])
+def test_slice_inverse_fp32_3m(n, b):
+    """The FP32 inverse with three real products per complex product (the
+    default of the mixed path) at the accuracy of the four-product kernels."""
+    rng = np.random.default_rng(n + b + 7)
+    A = _well_posed(rng, n)
+    X = rgf_amd.dense_inverse(A, "fp32_refined", b, complex_3m=True)
+    assert np.abs(X @ A - np.eye(n)).max() < 5e-5
+
+
+def test_dense_inverse_3m_flag_needs_fp32():
+    # invalid arguments come back as ValueError (RGF_ERR_INVALID_ARGUMENT)
+    with pytest.raises(ValueError, match="FP32 kernels"):
+        rgf_amd.dense_inverse(np.eye(8, dtype=complex), "fp64", complex_3m=True)
+
+
 def test_slice_inverse_pivots_inside_block():
     # zero diagonal: needs row interchanges within the diagonal block
     n = 48
