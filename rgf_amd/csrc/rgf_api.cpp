@@ -53,6 +53,7 @@ struct rgf_context {
     int* d_status = nullptr;
     double* d_scratch = nullptr;
     std::vector<rgf_context*> workers;  // worker contexts of rgf_solve_batched
+    std::vector<hipEvent_t> timer_events;  // kept across solves (options.profile)
 };
 
 static std::string g_last_error;
@@ -207,57 +208,59 @@ void upload_stencil(rgf_context* c, const Channels& ch, Work& w) {
     RGF_HIP(hipStreamSynchronize(c->stream));
 }
 
+// Transverse mode tables f1 (n1 x n1) and f2 (n2 x n2), interleaved FP64,
+// on the device: phi((i1,i2),(a1,a2)) = f1[i1][a1] f2[i2][a2].
+struct ModeTables {
+    const double* f1;
+    const double* f2;
+};
+
+ModeTables upload_modes(rgf_context* c, const Channels& ch) {
+    double* df1 = dbuf<double>(c, "f1", 2 * ch.f1.size());
+    double* df2 = dbuf<double>(c, "f2", 2 * ch.f2.size());
+    // std::complex<double> is two contiguous doubles (re, im)
+    RGF_HIP(hipMemcpyAsync(df1, ch.f1.data(), ch.f1.size() * 16, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipMemcpyAsync(df2, ch.f2.data(), ch.f2.size() * 16, hipMemcpyHostToDevice, c->stream));
+    RGF_HIP(hipStreamSynchronize(c->stream));
+    return {df1, df2};
+}
+
 // G0 = phi diag(exp(i kl)) phi^H, FP64 (cal_smatrix_RGF.m:198-203)
-void lead_green(rgf_context* c, const Channels& ch, const Side& side, CMat<double> G0) {
+void lead_green(rgf_context* c, const Channels& ch, const ModeTables& f, const Side& side, CMat<double> G0) {
     const int64_t N = ch.N;
-    std::vector<double> f1(2 * ch.f1.size()), f2(2 * ch.f2.size()), g(2 * N);
-    for (size_t q = 0; q < ch.f1.size(); ++q) {
-        f1[2 * q] = ch.f1[q].real();
-        f1[2 * q + 1] = ch.f1[q].imag();
-    }
-    for (size_t q = 0; q < ch.f2.size(); ++q) {
-        f2[2 * q] = ch.f2[q].real();
-        f2[2 * q + 1] = ch.f2[q].imag();
-    }
+    std::vector<double> g(2 * N);
     for (int64_t a = 0; a < N; ++a) {
         const cd e = std::exp(cd(0, 1) * side.kl[a]);
         g[2 * a] = e.real();
         g[2 * a + 1] = e.imag();
     }
-    double* df1 = dbuf<double>(c, "f1", f1.size());
-    double* df2 = dbuf<double>(c, "f2", f2.size());
     double* dg = dbuf<double>(c, "gdiag", g.size());
-    RGF_HIP(hipMemcpyAsync(df1, f1.data(), f1.size() * 8, hipMemcpyHostToDevice, c->stream));
-    RGF_HIP(hipMemcpyAsync(df2, f2.data(), f2.size() * 8, hipMemcpyHostToDevice, c->stream));
     RGF_HIP(hipMemcpyAsync(dg, g.data(), g.size() * 8, hipMemcpyHostToDevice, c->stream));
     CMat<double> P1 = dmat<double>(c, "tmpA", N, N), P2 = dmat<double>(c, "tmpB", N, N);
-    fill_modes(c->stream, ch.n1, ch.n2, df1, df2, dg, P1, P2);
+    fill_modes(c->stream, ch.n1, ch.n2, f.f1, f.f2, dg, P1, P2);
     cgemm<double>(c->stream, N, N, N, P1, P2, G0, cd(1, 0), 0);
     RGF_HIP(hipStreamSynchronize(c->stream));
 }
 
-// B = phi(:, prop(in)) diag(sqrt_mu(in)) (N x n); C = diag(sqrt_mu(out)) phi(:, prop(out))^H
-// (cal_smatrix_RGF.m:189-195; sqrt_mu is not conjugated in C)
-std::vector<cd> make_B(const Channels& ch, const Side& s, const std::vector<int32_t>& idx) {
-    const int64_t N = ch.N, n = idx.size();
-    std::vector<cd> B((size_t)N * n);
+// B = phi(:, prop(in)) diag(sqrt_mu(in)) (N x n, rows = 0); C = diag(sqrt_mu(out))
+// phi(:, prop(out))^H (n x N, rows = 1) (cal_smatrix_RGF.m:189-195; sqrt_mu is
+// not conjugated in C).  Filled on the device from the mode tables.
+void mode_cols(rgf_context* c, const Channels& ch, const ModeTables& f, const Side& s,
+               const std::vector<int32_t>& idx, int rows, const std::string& name, CMat<double> D) {
+    const int64_t n = idx.size();
+    if (n == 0) return;
+    std::vector<double> cols(4 * n);
     for (int64_t q = 0; q < n; ++q) {
         const int64_t a = s.ind_prop[idx[q]];
-        const cd mu = s.sqrt_mu[idx[q]];
-        for (int64_t i = 0; i < N; ++i) B[i * n + q] = mode_value(ch, i, a) * mu;
+        cols[4 * q] = (double)(a % ch.n1);
+        cols[4 * q + 1] = (double)(a / ch.n1);
+        cols[4 * q + 2] = s.sqrt_mu[idx[q]].real();
+        cols[4 * q + 3] = s.sqrt_mu[idx[q]].imag();
     }
-    return B;
-}
-
-std::vector<cd> make_C(const Channels& ch, const Side& s, const std::vector<int32_t>& idx) {
-    const int64_t N = ch.N, n = idx.size();
-    std::vector<cd> C((size_t)n * N);
-    for (int64_t q = 0; q < n; ++q) {
-        const int64_t a = s.ind_prop[idx[q]];
-        const cd mu = s.sqrt_mu[idx[q]];
-        for (int64_t i = 0; i < N; ++i) C[q * N + i] = mu * std::conj(mode_value(ch, i, a));
-    }
-    return C;
+    double* dcols = dbuf<double>(c, name, cols.size());
+    RGF_HIP(hipMemcpyAsync(dcols, cols.data(), cols.size() * 8, hipMemcpyHostToDevice, c->stream));
+    fill_mode_cols(c->stream, ch.n1, ch.n2, f.f1, f.f2, n, dcols, rows, D);
+    RGF_HIP(hipStreamSynchronize(c->stream));
 }
 
 void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Selection& sel, Work& w,
@@ -290,13 +293,14 @@ void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Sel
     const int64_t N = w.N;
     upload_eps(c, p, w);
     upload_stencil(c, ch, w);
+    const ModeTables f = upload_modes(c, ch);
     w.G0L = dmat<double>(c, "G0L", N, N);
-    lead_green(c, ch, *w.sL, w.G0L);
+    lead_green(c, ch, f, *w.sL, w.G0L);
     if (ch.same_sides) {
         w.G0R = w.G0L;
     } else {
         w.G0R = dmat<double>(c, "G0R", N, N);
-        lead_green(c, ch, *w.sR, w.G0R);
+        lead_green(c, ch, f, *w.sR, w.G0R);
     }
     release(c, "tmpA");
     release(c, "tmpB");
@@ -304,10 +308,10 @@ void prepare(rgf_context* c, const rgf_problem* p, const Channels& ch, const Sel
     w.BR = dmat<double>(c, "BR", N, w.inR.size());
     w.CL = dmat<double>(c, "CL", w.outL.size(), N);
     w.CR = dmat<double>(c, "CR", w.outR.size(), N);
-    if (!w.inL.empty()) upload(c, make_B(ch, *w.sL, w.inL), N, w.inL.size(), w.BL);
-    if (!w.inR.empty()) upload(c, make_B(ch, *w.sR, w.inR), N, w.inR.size(), w.BR);
-    if (!w.outL.empty()) upload(c, make_C(ch, *w.sL, w.outL), w.outL.size(), N, w.CL);
-    if (!w.outR.empty()) upload(c, make_C(ch, *w.sR, w.outR), w.outR.size(), N, w.CR);
+    mode_cols(c, ch, f, *w.sL, w.inL, 0, "BL_cols", w.BL);
+    mode_cols(c, ch, f, *w.sR, w.inR, 0, "BR_cols", w.BR);
+    mode_cols(c, ch, f, *w.sL, w.outL, 1, "CL_cols", w.CL);
+    mode_cols(c, ch, f, *w.sR, w.outR, 1, "CR_cols", w.CR);
 }
 
 // split-K workspace for the thin GEMMs (elements of T)
@@ -735,7 +739,7 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
     Work w;
     std::unique_ptr<KernelTimer> timer;
     if (o.profile) {
-        timer.reset(new KernelTimer);
+        timer.reset(new KernelTimer(&c->timer_events));
         w.timer = timer.get();
     }
     prepare(c, p, ch, sel, w, r2l ? 1 : 0);
@@ -844,7 +848,7 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
     Work w;
     std::unique_ptr<KernelTimer> timer;
     if (o.profile) {
-        timer.reset(new KernelTimer);
+        timer.reset(new KernelTimer(&c->timer_events));
         w.timer = timer.get();
     }
     prepare(c, p, ch, sel, w);
@@ -984,6 +988,8 @@ rgf_status rgf_destroy(rgf_handle h) {
         if (h->own) RGF_HIP(hipStreamDestroy(h->own));
         for (auto& e : h->ov.ev)
             if (e) RGF_HIP(hipEventDestroy(e));
+        for (auto e : h->timer_events) RGF_HIP(hipEventDestroy(e));
+        h->timer_events.clear();
         if (h->ov.s2) RGF_HIP(hipStreamDestroy(h->ov.s2));
         if (h->ov.s3) RGF_HIP(hipStreamDestroy(h->ov.s3));
     });

@@ -187,11 +187,6 @@ Channels setup_channels(const rgf_problem* p) {
     return ch;
 }
 
-cd mode_value(const Channels& ch, int64_t i, int64_t a) {
-    const int64_t i1 = i % ch.n1, i2 = i / ch.n1, a1 = a % ch.n1, a2 = a / ch.n1;
-    return ch.f1[i1 * ch.n1 + a1] * ch.f2[i2 * ch.n2 + a2];
-}
-
 // build_laplacian_1d.m:284-313 as the sparse product -grad1^H grad1; entries
 // with equal (row, col) accumulate, as MATLAB's sparse() does (this fixes the
 // Bloch n = 1 case to -4 sin^2(ka/2)).
@@ -237,34 +232,47 @@ std::map<std::pair<int64_t, int64_t>, cd> laplacian_1d(int64_t n, int bc, double
     return A;
 }
 
-// A0 = 2I - kron(I, L1) - kron(L2, I) (cal_smatrix_RGF.m:209; _3D.m:218-224)
+// A0 = 2I - kron(I, L1) - kron(L2, I) (cal_smatrix_RGF.m:209; _3D.m:218-224).
+// Each row holds a handful of entries: they are gathered per row in the
+// order the terms are added (the diagonal, then L1, then L2, as sparse()
+// sums them) and sorted by column at the end.
 HostCSR slice_stencil(const Channels& ch) {
-    std::map<std::pair<int64_t, int64_t>, cd> A;
     const int64_t n1 = ch.n1, n2 = ch.n2, N = ch.N;
-    for (int64_t i = 0; i < N; ++i) A[{i, i}] += cd(2, 0);
+    std::vector<std::vector<std::pair<int64_t, cd>>> rows(N);
+    auto add = [&](int64_t r, int64_t col, cd v) {
+        for (auto& e : rows[r])
+            if (e.first == col) {
+                e.second += v;
+                return;
+            }
+        rows[r].push_back({col, cd(0, 0) + v});
+    };
+    for (int64_t i = 0; i < N; ++i) add(i, i, cd(2, 0));
     auto L1 = laplacian_1d(n1, ch.bc[0], ch.ka[0]);
     for (int64_t i2 = 0; i2 < n2; ++i2)
-        for (auto& e : L1) A[{e.first.first + n1 * i2, e.first.second + n1 * i2}] -= e.second;
+        for (auto& e : L1) add(e.first.first + n1 * i2, e.first.second + n1 * i2, -e.second);
     if (ch.ndim == 3) {
         auto L2 = laplacian_1d(n2, ch.bc[1], ch.ka[1]);
         for (int64_t i1 = 0; i1 < n1; ++i1)
-            for (auto& e : L2) A[{i1 + n1 * e.first.first, i1 + n1 * e.first.second}] -= e.second;
+            for (auto& e : L2) add(i1 + n1 * e.first.first, i1 + n1 * e.first.second, -e.second);
     }
     HostCSR c;
     c.n = (int32_t)N;
     c.rowptr.assign(N + 1, 0);
-    for (auto& e : A) c.rowptr[e.first.first + 1]++;
-    for (int64_t i = 0; i < N; ++i) c.rowptr[i + 1] += c.rowptr[i];
-    c.col.resize(A.size());
-    c.vre.resize(A.size());
-    c.vim.resize(A.size());
-    std::vector<int32_t> pos(c.rowptr.begin(), c.rowptr.end() - 1);
-    for (auto& e : A) {
-        const int32_t q = pos[e.first.first]++;
-        c.col[q] = (int32_t)e.first.second;
-        c.vre[q] = e.second.real();
-        c.vim[q] = e.second.imag();
+    for (int64_t i = 0; i < N; ++i) {
+        std::sort(rows[i].begin(), rows[i].end(),
+                  [](const std::pair<int64_t, cd>& a, const std::pair<int64_t, cd>& b) { return a.first < b.first; });
+        c.rowptr[i + 1] = c.rowptr[i] + (int32_t)rows[i].size();
     }
+    c.col.reserve(c.rowptr[N]);
+    c.vre.reserve(c.rowptr[N]);
+    c.vim.reserve(c.rowptr[N]);
+    for (auto& r : rows)
+        for (auto& e : r) {
+            c.col.push_back((int32_t)e.first);
+            c.vre.push_back(e.second.real());
+            c.vim.push_back(e.second.imag());
+        }
     return c;
 }
 

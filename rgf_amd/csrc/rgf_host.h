@@ -45,7 +45,6 @@ struct Selection {                  // 0-based channel indices, user order
 cd kx_from_z(cd z);
 void validate_problem(const rgf_problem* p);
 Channels setup_channels(const rgf_problem* p);
-cd mode_value(const Channels& ch, int64_t i, int64_t a);
 std::map<std::pair<int64_t, int64_t>, cd> laplacian_1d(int64_t n, int bc, double ka);
 HostCSR slice_stencil(const Channels& ch);
 Selection make_selection(const rgf_selection* s, const Channels& ch);

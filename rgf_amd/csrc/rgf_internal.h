@@ -54,8 +54,15 @@ struct DevCSR {
 // Times one launch in every `stride` with a pair of HIP events on its
 // stream (events add latency to the stream); flops and launches count the
 // timed launches only, so flops / seconds estimates the kernel's rate.
+// The events come from `pool` when one is given (kept by the caller across
+// timers: creating and destroying a few thousand events per solve costs
+// milliseconds of host time), else from the timer's own list.
 struct KernelTimer {
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> own;
+    std::vector<hipEvent_t>& ev;
+    explicit KernelTimer(std::vector<hipEvent_t>* pool = nullptr) : ev(pool ? *pool : own) {}
+    KernelTimer(const KernelTimer&) = delete;
+    KernelTimer& operator=(const KernelTimer&) = delete;
     size_t used = 0;
     double flops = 0;
     int64_t launches = 0;  // timed launches
@@ -124,6 +131,11 @@ void add_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> a, CMat<T> b, CM
 // (row-major interleaved FP64), g length N interleaved (may be null -> 1).
 void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2,
                 const double* g, CMat<double> P1, CMat<double> P2);
+// Selected mode columns q = 0..n-1 of phi, scaled: cols holds (a1, a2, s_re,
+// s_im) per column (doubles).  rows = 0: D(i, q) = phi(i, a_q) s_q (N x n);
+// rows = 1: D(q, i) = s_q conj(phi(i, a_q)) (n x N).
+void fill_mode_cols(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2, int64_t n,
+                    const double* cols, int rows, CMat<double> D);
 
 // One forward-sweep product of the mixed refinement: D = G t (FP32; G N x N,
 // t and D N x m); with `next`, also t = (float)(sc * next) + D, the right-hand

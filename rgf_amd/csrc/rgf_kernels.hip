@@ -6,7 +6,8 @@
 //                      trailing updates run on MFMA through cgemm.
 //  K3 cgemm            complex GEMM on MFMA (v_mfma_f64_16x16x4 /
 //                      v_mfma_f32_16x16x4), planar re/im operands, 4M or 3M.
-//  K4 fill_modes       transverse mode matrices for the lead Green's function.
+//  K4 fill_modes       transverse mode matrices for the lead Green's function,
+//     fill_mode_cols   and the selected mode columns of the lead projections.
 //  K5 (projection)     Fisher-Lee products are cgemm calls from the driver.
 //  residual_stencil    FP64 residual of the block-tridiagonal system used by
 //                      the mixed-precision refinement.
@@ -1847,7 +1848,7 @@ double KernelTimer::seconds() {
 }
 
 KernelTimer::~KernelTimer() {
-    for (auto e : ev) (void)hipEventDestroy(e);
+    for (auto e : own) (void)hipEventDestroy(e);
 }
 
 template <typename T>
@@ -2239,6 +2240,41 @@ void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const d
     const int64_t N = n1 * n2;
     hipLaunchKernelGGL(fill_modes_kernel, dim3(grid_for(N * N)), dim3(256), 0, s, n1, n2, f1, f2, g, P1.re, P1.im,
                        P1.ld, P2.re, P2.im, P2.ld);
+    RGF_HIP(hipGetLastError());
+}
+
+// Selected mode columns: cols holds, per column q, (a1, a2, s_re, s_im) as
+// doubles; phi_q(i) = f1[i1][a1] f2[i2][a2] with i = i1 + n1 i2.  rows = 0:
+// D(i, q) = phi_q(i) s_q (N x n); rows = 1: D(q, i) = s_q conj(phi_q(i)) (n x N).
+__global__ void fill_mode_cols_kernel(int64_t n1, int64_t n2, const double* f1, const double* f2, int64_t n,
+                                      const double* cols, int rows, double* dr, double* di, int64_t ld) {
+    const int64_t N = n1 * n2;
+    const int64_t total = N * n;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = rows ? e / N : e % n, i = rows ? e % N : e / n;
+        const int64_t i1 = i % n1, i2 = i / n1;
+        const int64_t a1 = (int64_t)cols[4 * q], a2 = (int64_t)cols[4 * q + 1];
+        const double sr = cols[4 * q + 2], si = cols[4 * q + 3];
+        const double xr = f1[2 * (i1 * n1 + a1)], xi = f1[2 * (i1 * n1 + a1) + 1];
+        const double yr = f2[2 * (i2 * n2 + a2)], yi = f2[2 * (i2 * n2 + a2) + 1];
+        const double pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
+        if (rows) {
+            dr[q * ld + i] = sr * pr + si * pi;  // s conj(phi)
+            di[q * ld + i] = si * pr - sr * pi;
+        } else {
+            dr[i * ld + q] = pr * sr - pi * si;
+            di[i * ld + q] = pr * si + pi * sr;
+        }
+    }
+}
+
+void fill_mode_cols(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2, int64_t n,
+                    const double* cols, int rows, CMat<double> D) {
+    const int64_t N = n1 * n2;
+    if (N == 0 || n == 0) return;
+    hipLaunchKernelGGL(fill_mode_cols_kernel, dim3(grid_for(N * n)), dim3(256), 0, s, n1, n2, f1, f2, n, cols, rows,
+                       D.re, D.im, D.ld);
     RGF_HIP(hipGetLastError());
 }
 
