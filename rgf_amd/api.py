@@ -92,6 +92,7 @@ class _Prob:
 
     def __init__(self, ndim, dims, eps, epsL, epsR, k0dx, bcs, kas):
         self.keep = []
+        self.device = None  # device index of device-resident eps
         p = L.Problem()
         p.ndim = ndim
         p.dims[:] = list(dims)
@@ -112,12 +113,67 @@ class _Prob:
         self.p = p
 
 
+def _is_tensor(x) -> bool:
+    return type(x).__module__.split(".")[0] == "torch"
+
+
+def _problem_from_tensor(eps, epsilon_L, epsilon_R, k0dx, bc, three_d: bool) -> _Prob:
+    """rgf_problem over a torch tensor in MATLAB index order (ny x nx, or
+    nx x ny x nz).  A CUDA tensor is handed over as device pointers
+    (rgf_problem.eps_on_device = 1): the column-major FP64 planes are formed
+    on the device and torch's current stream is synchronized before the
+    library reads them from its own stream."""
+    import torch
+    if not (eps.is_floating_point() or eps.is_complex()):
+        raise ValueError("syst.epsilon must be a numeric array")
+    if three_d:
+        if eps.dim() == 2:
+            eps = eps[:, :, None]
+        if eps.dim() != 3:
+            raise ValueError("syst.epsilon must be a 3D array")
+        nx, ny, nz = eps.shape
+        if nx * ny == 0:
+            raise ValueError("need at least one site along the transverse directions (x,y)")
+        dims = (nx, ny, nz)
+        bcs, kas = _bc_xy(bc)
+        cm = eps.detach().permute(2, 1, 0)            # column-major view: slices contiguous
+    else:
+        if eps.dim() == 1:
+            eps = eps[:, None]
+        if eps.dim() != 2:
+            raise ValueError("syst.epsilon must be a numeric matrix")
+        ny, nx = eps.shape
+        if ny == 0:
+            raise ValueError("need at least one site along the transverse (y) direction")
+        dims = (ny, nx, 1)
+        b, ka = _bc_1d(bc)
+        bcs, kas = [b], [ka]
+        cm = eps.detach().t()
+    prob = _Prob(3 if three_d else 2, dims, None, _scalar(epsilon_L, "epsilon_L"), _scalar(epsilon_R, "epsilon_R"),
+                 _scalar(k0dx, "k0dx"), bcs, kas)
+    re = cm.real.to(torch.float64).contiguous() if cm.is_complex() else cm.to(torch.float64).contiguous()
+    prob.keep.append(re)
+    prob.p.eps_re = re.data_ptr()
+    if cm.is_complex():
+        im = cm.imag.to(torch.float64).contiguous()
+        prob.keep.append(im)
+        prob.p.eps_im = im.data_ptr()
+    prob.p.eps_on_device = 1 if eps.is_cuda else 0
+    if eps.is_cuda:
+        prob.device = eps.device.index
+        torch.cuda.current_stream(eps.device).synchronize()
+    return prob
+
+
 def _problem_2d(syst, need_eps=True) -> _Prob:
     if not isinstance(syst, dict):
         raise ValueError("syst must be a structure")
     for key in ("epsilon", "epsilon_L", "epsilon_R", "k0dx", "yBC"):
         if key not in syst:
             raise ValueError(f"syst.{key} must be provided")
+    if _is_tensor(syst["epsilon"]):
+        return _problem_from_tensor(syst["epsilon"], syst["epsilon_L"], syst["epsilon_R"], syst["k0dx"], syst["yBC"],
+                                    False)
     eps = np.asarray(syst["epsilon"])
     if eps.ndim == 1:
         eps = eps[:, None]
@@ -137,6 +193,9 @@ def _problem_3d(syst) -> _Prob:
     for key in ("epsilon", "epsilon_L", "epsilon_R", "k0dx", "xyBC"):
         if key not in syst:
             raise ValueError(f"syst.{key} must be provided")
+    if _is_tensor(syst["epsilon"]):
+        return _problem_from_tensor(syst["epsilon"], syst["epsilon_L"], syst["epsilon_R"], syst["k0dx"],
+                                    syst["xyBC"], True)
     eps = np.asarray(syst["epsilon"])
     if eps.ndim == 2:
         eps = eps[:, :, None]
@@ -237,6 +296,10 @@ def _solve(prob: _Prob, out, in_, precision, device, opts, fields: bool = False)
         if fields:
             return S0, None, np.zeros((N, nslab, nin.value), dtype=np.complex128, order="F")
         return S0, None
+    if prob.device is not None:
+        if device is not None and device >= 0 and device != prob.device:
+            raise ValueError(f"syst.epsilon is on device {prob.device}, the solve was asked for device {device}")
+        device = prob.device
     h = L.handle(device)
     S = np.zeros((nout.value, nin.value), dtype=np.complex128, order="F")
     info = L.Info()

@@ -13,33 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .api import _bc_1d, _bc_xy, _options, _parse_selection, _Prob, _query, _scalar, _Sel
-
-
-def _problem_from_tensor(eps: torch.Tensor, epsilon_L, epsilon_R, k0dx, bc, three_d: bool):
-    """rgf_problem over a tensor in MATLAB index order (ny x nx, or nx x ny x nz)."""
-    if three_d:
-        nx, ny, nz = eps.shape
-        dims = (nx, ny, nz)
-        bcs, kas = _bc_xy(bc)
-        cm = eps.detach().permute(2, 1, 0)            # column-major view: slices contiguous
-    else:
-        ny, nx = eps.shape
-        dims = (ny, nx, 1)
-        b, ka = _bc_1d(bc)
-        bcs, kas = [b], [ka]
-        cm = eps.detach().t()
-    prob = _Prob(3 if three_d else 2, dims, None, _scalar(epsilon_L, "epsilon_L"), _scalar(epsilon_R, "epsilon_R"),
-                 _scalar(k0dx, "k0dx"), bcs, kas)
-    re = cm.real.to(torch.float64).contiguous() if cm.is_complex() else cm.to(torch.float64).contiguous()
-    prob.keep.append(re)
-    prob.p.eps_re = re.data_ptr()
-    if cm.is_complex():
-        im = cm.imag.to(torch.float64).contiguous()
-        prob.keep.append(im)
-        prob.p.eps_im = im.data_ptr()
-    prob.p.eps_on_device = 1 if eps.is_cuda else 0
-    return prob
+from .api import _options, _parse_selection, _problem_from_tensor, _query, _Sel
 
 
 class TransmissionFunction(torch.autograd.Function):

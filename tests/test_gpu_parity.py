@@ -178,6 +178,42 @@ def test_struct_selection_and_delta():
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
+@pytest.mark.parametrize("where", ["cuda", "cpu"])
+def test_epsilon_as_tensor(precision, tol, where):
+    """syst.epsilon given as a torch tensor (device pointers for a CUDA
+    tensor, host pointers for a CPU one) gives the S of the numpy input:
+    2D complex with the sweep mirrored or not, 3D from a non-contiguous view."""
+    import torch
+    rng = np.random.default_rng(77)
+    eps2 = 1 + 2 * rng.random((37, 23)) + 0.02j * rng.random((37, 23))
+    syst = dict(epsilon=eps2, epsilon_L=1.0, epsilon_R=1.69, k0dx=2 * np.pi / 6.5, yBC="Neumann")
+    tsyst = dict(syst, epsilon=torch.as_tensor(eps2, device=where))
+    for out, inn in ((["L", "R"], ["L"]), (["L"], ["R"])):
+        S, _ = rgf_amd.cal_smatrix_RGF(syst, out, inn, precision=precision)
+        St, _ = rgf_amd.cal_smatrix_RGF(tsyst, out, inn, precision=precision)
+        assert np.abs(St - S).max() <= 1e-13
+        Sref, _ = o.cal_smatrix_RGF(syst, out, inn)
+        assert np.abs(St - Sref).max() < tol
+    e3 = 1 + 1.1 * rng.random((7, 5, 6))
+    t3 = torch.as_tensor(e3, device=where).permute(2, 1, 0)        # (6, 5, 7), not contiguous
+    syst3 = dict(epsilon=np.ascontiguousarray(e3.transpose(2, 1, 0)), epsilon_L=1.0, epsilon_R=1.3, k0dx=1.2,
+                 xyBC=[0.7, 2.9])
+    S, _ = rgf_amd.cal_smatrix_RGF_3D(syst3, ["L", "R"], ["L", "R"], precision=precision)
+    St, _ = rgf_amd.cal_smatrix_RGF_3D(dict(syst3, epsilon=t3), ["L", "R"], ["L", "R"], precision=precision)
+    assert np.abs(St - S).max() <= 1e-13
+    Sref, _ = o.cal_smatrix_RGF_3D(syst3, ["L", "R"], ["L", "R"])
+    assert np.abs(St - Sref).max() < tol
+
+
+def test_epsilon_tensor_device_mismatch():
+    import torch
+    syst = dict(epsilon=torch.ones((8, 5), dtype=torch.float64, device="cuda:0"), epsilon_L=1.0, epsilon_R=1.0,
+                k0dx=1.0, yBC="periodic")
+    with pytest.raises(ValueError, match="is on device 0"):
+        rgf_amd.cal_smatrix_RGF(syst, ["R"], ["L"], device=1)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
 def test_smatrix_3d(precision, tol):
     rng = np.random.default_rng(21)
     for bc in ["periodic", "Dirichlet", [0.7, 2.9]]:
