@@ -16,6 +16,8 @@ DRIVER = os.path.join(HERE, "rgf_inverse_driver")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["rgf_kernels.hip", "rgf_channels.cpp", "rgf_api.cpp"]
+MEX_GATEWAY = os.path.join(HERE, "mex", "cal_smatrix_RGF_mex.c")
+MEX_HARNESS = os.path.join(os.path.dirname(HERE), "tests", "mex_harness")
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in device code.
 HIPCC_FLAGS = [
@@ -60,7 +62,24 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(DRIVER + ".tmp", DRIVER)
+    build_mex_harness(verbose)
     return LIB
+
+
+def build_mex_harness(verbose: bool = False) -> str | None:
+    """The MEX gateway compiled against the tests' host implementation of the
+    MEX API (tests/mex_harness), so that it runs without MATLAB."""
+    if not os.path.isdir(MEX_HARNESS):
+        return None
+    out = os.path.join(MEX_HARNESS, "libmexgw.so")
+    cmd = ["gcc", "-O2", "-std=gnu11", "-Wall", "-shared", "-fPIC", "-I", MEX_HARNESS, "-I", INCLUDE, MEX_GATEWAY,
+           os.path.join(MEX_HARNESS, "mex_harness.c"), "-o", out + ".tmp", "-L", HERE, "-lrgf",
+           "-Wl,-rpath,$ORIGIN/../../rgf_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -26,6 +26,7 @@
  *   mkoctfile --mex -I../../include cal_smatrix_RGF_mex.c -L.. -lrgf
  */
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -207,7 +208,7 @@ static void read_side(const mxArray* spec, const char* kind, int32_t** ind_L, in
         double* src = mxIsDouble(v) ? mxGetDoubles(v) : NULL;
         for (size_t q = 0; q < n; ++q) {
             double x = src ? src[q] : 0.0;
-            if (!src || x != (double)(int32_t)x) {
+            if (!src || !(x >= -2147483648.0 && x <= 2147483647.0) || x != (double)(int32_t)x) {
                 /* non-integers are rejected like out-of-range indices (cal_smatrix_RGF.m:110) */
                 a[q] = 0;
             } else {
@@ -245,12 +246,12 @@ static void get_opts(const mxArray* s, rgf_options* o, int32_t* every) {
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     int gradient = nrhs >= 1 && mxIsChar(prhs[0]);
     int a0 = gradient ? 1 : 0;
-    if (nrhs < a0 + 3) fail("RGF:input", "usage: cal_smatrix_RGF_mex(syst, out, in [, opts])");
-    if (!g_handle) {
-        rgf_status st = rgf_create(&g_handle, -1);
-        if (st != RGF_OK) fail("RGF:device", rgf_last_error(NULL));
-        mexAtExit(release_handle);
+    if (gradient) {
+        char mode[16] = {0};
+        mxGetString(prhs[0], mode, sizeof mode);
+        if (strcmp(mode, "gradient")) fail("RGF:input", "the only mode argument is 'gradient'");
     }
+    if (nrhs < a0 + 3) fail("RGF:input", "usage: cal_smatrix_RGF_mex(syst, out, in [, opts])");
     rgf_problem p;
     eps_buf eb;
     read_syst(prhs[a0], &p, &eb);
@@ -264,36 +265,45 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     int32_t every;
     get_opts(nrhs > a0 + 3 ? prhs[a0 + 3] : NULL, &opt, &every);
     rgf_info info;
-    if (!gradient) {
-        int32_t nL, nR, nout, nin;
-        check(rgf_query(&p, &sel, &nL, &nR, &nout, &nin), NULL);
-        if (nout == 0 || nin == 0) {
-            plhs[0] = mxCreateDoubleMatrix(nout, nin, mxREAL); /* cal_smatrix_RGF.m:181-184 */
-        } else {
+    memset(&info, 0, sizeof info);
+    int32_t nL, nR, nout, nin;
+    /* the problem and the selection are checked before any device is touched */
+    check(rgf_query(&p, &sel, &nL, &nR, &nout, &nin), NULL);
+    info.n_prop_L = nL;
+    info.n_prop_R = nR;
+    if (!gradient && (nout == 0 || nin == 0)) {
+        plhs[0] = mxCreateDoubleMatrix(nout, nin, mxREAL); /* cal_smatrix_RGF.m:181-184 */
+    } else {
+        if (!g_handle) {
+            rgf_status st = rgf_create(&g_handle, -1);
+            if (st != RGF_OK) fail("RGF:device", rgf_last_error(NULL));
+            mexAtExit(release_handle);
+        }
+        if (!gradient) {
             plhs[0] = mxCreateDoubleMatrix(nout, nin, mxCOMPLEX);
             check(rgf_solve(g_handle, &p, &sel, &opt, (double*)mxGetComplexDoubles(plhs[0]), &info), g_handle);
-        }
-    } else {
-        size_t n = (size_t)(p.dims[0] * p.dims[1] * p.dims[2]);
-        double T = 0;
-        mxArray* g = mxCreateNumericArray(p.ndim == 3 ? 3 : 2, (const mwSize[]){(mwSize)p.dims[0], (mwSize)p.dims[1], (mwSize)p.dims[2]}, mxDOUBLE_CLASS, p.eps_im ? mxCOMPLEX : mxREAL);
-        double* gre = (double*)mxMalloc(n * sizeof(double));
-        double* gim = (double*)mxMalloc(n * sizeof(double));
-        check(rgf_transmission_gradient(g_handle, &p, &sel, &opt, every, &T, gre, gim, &info), g_handle);
-        if (p.eps_im) {
-            mxComplexDouble* z = mxGetComplexDoubles(g);
-            for (size_t k = 0; k < n; ++k) {
-                z[k].real = gre[k];
-                z[k].imag = gim[k];
-            }
         } else {
-            memcpy(mxGetDoubles(g), gre, n * sizeof(double));
+            size_t n = (size_t)(p.dims[0] * p.dims[1] * p.dims[2]);
+            double T = 0;
+            mxArray* g = mxCreateNumericArray(p.ndim == 3 ? 3 : 2, (const mwSize[]){(mwSize)p.dims[0], (mwSize)p.dims[1], (mwSize)p.dims[2]}, mxDOUBLE_CLASS, p.eps_im ? mxCOMPLEX : mxREAL);
+            double* gre = (double*)mxMalloc(n * sizeof(double));
+            double* gim = (double*)mxMalloc(n * sizeof(double));
+            check(rgf_transmission_gradient(g_handle, &p, &sel, &opt, every, &T, gre, gim, &info), g_handle);
+            if (p.eps_im) {
+                mxComplexDouble* z = mxGetComplexDoubles(g);
+                for (size_t k = 0; k < n; ++k) {
+                    z[k].real = gre[k];
+                    z[k].imag = gim[k];
+                }
+            } else {
+                memcpy(mxGetDoubles(g), gre, n * sizeof(double));
+            }
+            mxFree(gre);
+            mxFree(gim);
+            plhs[0] = mxCreateDoubleScalar(T);
+            if (nlhs > 1) plhs[1] = g;
+            else mxDestroyArray(g);
         }
-        mxFree(gre);
-        mxFree(gim);
-        plhs[0] = mxCreateDoubleScalar(T);
-        if (nlhs > 1) plhs[1] = g;
-        else mxDestroyArray(g);
     }
     if (!gradient && nlhs > 1) {
         const char* f[] = {"n_prop_L", "n_prop_R", "refine_iters", "refine_residual", "seconds_total"};
