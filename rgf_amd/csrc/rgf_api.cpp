@@ -236,9 +236,15 @@ void lead_green(rgf_context* c, const Channels& ch, const ModeTables& f, const S
     }
     double* dg = dbuf<double>(c, "gdiag", g.size());
     RGF_HIP(hipMemcpyAsync(dg, g.data(), g.size() * 8, hipMemcpyHostToDevice, c->stream));
-    CMat<double> P1 = dmat<double>(c, "tmpA", N, N), P2 = dmat<double>(c, "tmpB", N, N);
-    fill_modes(c->stream, ch.n1, ch.n2, f.f1, f.f2, dg, P1, P2);
-    cgemm<double>(c->stream, N, N, N, P1, P2, G0, cd(1, 0), 0);
+    if (ch.n2 > 1) {
+        // 3D lead: phi = f1 (x) f2, two contractions (N^2 (n1 + n2) work, no N x N temporaries)
+        double* M = dbuf<double>(c, "g0_stage", (size_t)2 * ch.n1 * ch.n1 * ch.n2);
+        lead_green_separable(c->stream, ch.n1, ch.n2, f.f1, f.f2, dg, M, G0);
+    } else {
+        CMat<double> P1 = dmat<double>(c, "tmpA", N, N), P2 = dmat<double>(c, "tmpB", N, N);
+        fill_modes(c->stream, ch.n1, ch.n2, f.f1, f.f2, dg, P1, P2);
+        cgemm<double>(c->stream, N, N, N, P1, P2, G0, cd(1, 0), 0);
+    }
     RGF_HIP(hipStreamSynchronize(c->stream));
 }
 

@@ -131,6 +131,11 @@ void add_mat(hipStream_t s, int64_t rows, int64_t cols, CMat<T> a, CMat<T> b, CM
 // (row-major interleaved FP64), g length N interleaved (may be null -> 1).
 void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2,
                 const double* g, CMat<double> P1, CMat<double> P2);
+// G0 = phi diag(g) phi^H for phi = f1 (x) f2 (n1 x n1 and n2 x n2 tables,
+// g length n1 n2, interleaved FP64) in two contractions; M is scratch of
+// 2 n1^2 n2 doubles.
+void lead_green_separable(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2, const double* g,
+                          double* M, CMat<double> G0);
 // Selected mode columns q = 0..n-1 of phi, scaled: cols holds (a1, a2, s_re,
 // s_im) per column (doubles).  rows = 0: D(i, q) = phi(i, a_q) s_q (N x n);
 // rows = 1: D(q, i) = s_q conj(phi(i, a_q)) (n x N).

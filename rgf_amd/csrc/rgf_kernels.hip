@@ -6,8 +6,10 @@
 //                      trailing updates run on MFMA through cgemm.
 //  K3 cgemm            complex GEMM on MFMA (v_mfma_f64_16x16x4 /
 //                      v_mfma_f32_16x16x4), planar re/im operands, 4M or 3M.
-//  K4 fill_modes       transverse mode matrices for the lead Green's function,
-//     fill_mode_cols   and the selected mode columns of the lead projections.
+//  K4 fill_modes       transverse mode matrices for the lead Green's function
+//                      (lead_green_separable: the same in two contractions
+//                      for a 3D lead), fill_mode_cols: the selected mode
+//                      columns of the lead projections.
 //  K5 (projection)     Fisher-Lee products are cgemm calls from the driver.
 //  residual_stencil    FP64 residual of the block-tridiagonal system used by
 //                      the mixed-precision refinement.
@@ -2240,6 +2242,63 @@ void fill_modes(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const d
     const int64_t N = n1 * n2;
     hipLaunchKernelGGL(fill_modes_kernel, dim3(grid_for(N * N)), dim3(256), 0, s, n1, n2, f1, f2, g, P1.re, P1.im,
                        P1.ld, P2.re, P2.im, P2.ld);
+    RGF_HIP(hipGetLastError());
+}
+
+// Lead Green's function of a separable lead, G0 = phi diag(g) phi^H with
+// phi = f1 (x) f2, in two contractions instead of an N^3 product:
+//   M[a2][i1][j1] = sum_a1 f1[i1][a1] g[a1 + n1 a2] conj(f1[j1][a1])
+//   G0[i][j]      = sum_a2 f2[i2][a2] M[a2][i1][j1] conj(f2[j2][a2])
+// with i = i1 + n1 i2, j = j1 + n1 j2 (n1^3 n2 + N^2 n2 multiply-adds).
+__global__ void lead_green_stage1_kernel(int64_t n1, int64_t n2, const double* f1, const double* g, double* M) {
+    const int64_t total = n2 * n1 * n1;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j1 = e % n1, i1 = (e / n1) % n1, a2 = e / (n1 * n1);
+        double sr = 0, si = 0;
+        for (int64_t a1 = 0; a1 < n1; ++a1) {
+            const double xr = f1[2 * (i1 * n1 + a1)], xi = f1[2 * (i1 * n1 + a1) + 1];
+            const double yr = f1[2 * (j1 * n1 + a1)], yi = -f1[2 * (j1 * n1 + a1) + 1];
+            const double gr = g[2 * (a1 + n1 * a2)], gi = g[2 * (a1 + n1 * a2) + 1];
+            const double tr = xr * gr - xi * gi, ti = xr * gi + xi * gr;
+            sr += tr * yr - ti * yi;
+            si += tr * yi + ti * yr;
+        }
+        M[2 * e] = sr;
+        M[2 * e + 1] = si;
+    }
+}
+
+__global__ void lead_green_stage2_kernel(int64_t n1, int64_t n2, const double* f2, const double* M, double* Gre,
+                                         double* Gim, int64_t ld) {
+    const int64_t N = n1 * n2;
+    for (int64_t i = blockIdx.y; i < N; i += gridDim.y) {
+        const int64_t i1 = i % n1, i2 = i / n1;
+        for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
+            const int64_t j1 = j % n1, j2 = j / n1;
+            double sr = 0, si = 0;
+            for (int64_t a2 = 0; a2 < n2; ++a2) {
+                const double xr = f2[2 * (i2 * n2 + a2)], xi = f2[2 * (i2 * n2 + a2) + 1];
+                const double yr = f2[2 * (j2 * n2 + a2)], yi = -f2[2 * (j2 * n2 + a2) + 1];
+                const double* m = M + 2 * ((a2 * n1 + i1) * n1 + j1);
+                const double tr = xr * m[0] - xi * m[1], ti = xr * m[1] + xi * m[0];
+                sr += tr * yr - ti * yi;
+                si += tr * yi + ti * yr;
+            }
+            Gre[i * ld + j] = sr;
+            Gim[i * ld + j] = si;
+        }
+    }
+}
+
+void lead_green_separable(hipStream_t s, int64_t n1, int64_t n2, const double* f1, const double* f2, const double* g,
+                          double* M, CMat<double> G0) {
+    const int64_t N = n1 * n2;
+    if (N == 0) return;
+    hipLaunchKernelGGL(lead_green_stage1_kernel, dim3(grid_for(n2 * n1 * n1)), dim3(256), 0, s, n1, n2, f1, g, M);
+    const unsigned gx = (unsigned)std::min<int64_t>((N + 255) / 256, 64), gy = (unsigned)std::min<int64_t>(N, 65535);
+    hipLaunchKernelGGL(lead_green_stage2_kernel, dim3(gx, gy), dim3(256), 0, s, n1, n2, f2, (const double*)M, G0.re,
+                       G0.im, G0.ld);
     RGF_HIP(hipGetLastError());
 }
 
