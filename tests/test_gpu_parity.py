@@ -226,6 +226,22 @@ def test_smatrix_3d(precision, tol):
             assert np.abs(S - Sref).max() < tol
 
 
+@pytest.mark.parametrize("shape", [(1, 6, 4), (6, 1, 4), (1, 1, 3), (9, 2, 3), (2, 9, 3)])
+@pytest.mark.parametrize("bc", ["periodic", "Dirichlet", [1.3, -0.4]])
+def test_smatrix_3d_thin_transverse(shape, bc):
+    """Transverse shapes where one direction has one or two sites: the lead
+    Green's function goes through the separable two-contraction form when
+    ny > 1 and through the dense product when ny = 1."""
+    rng = np.random.default_rng(sum(shape) + len(str(bc)))
+    eps = 1 + 1.5 * rng.random(shape) + 0.01j * rng.random(shape)
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.44, k0dx=2.2, xyBC=bc)
+    S, _ = rgf_amd.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"])
+    Sref, _ = o.cal_smatrix_RGF_3D(syst, ["L", "R"], ["L", "R"])
+    assert S.shape == Sref.shape
+    if S.size:
+        assert np.abs(S - Sref).max() < TOL64
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_mixed_3m_modes(mode):
     """Three-product complex GEMMs in the slice inverses (bit 0) and/or the
