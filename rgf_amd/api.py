@@ -124,7 +124,7 @@ def _problem_from_tensor(eps, epsilon_L, epsilon_R, k0dx, bc, three_d: bool) -> 
     on the device and torch's current stream is synchronized before the
     library reads them from its own stream."""
     import torch
-    if not (eps.is_floating_point() or eps.is_complex()):
+    if eps.dtype == torch.bool:
         raise ValueError("syst.epsilon must be a numeric array")
     if three_d:
         if eps.dim() == 2:
