@@ -8,8 +8,8 @@
  *   channels = setup_channels(ny, BC, k0dx, eps_L, eps_R)     src/setup_channels.m:1
  *   channels = setup_channels_3D(nx, ny, BC, k0dx, eps_L, eps_R)  src/setup_channels_3D.m:1
  *   A = build_laplacian_1d(n, BC)                             src/build_laplacian_1d.m:1
- * and the entry points below are what a MEX gateway (rgf_amd/mex/rgf_mex.c)
- * or a ctypes binding binds to reproduce them.
+ * and the entry points below are what a MEX gateway
+ * (rgf_amd/mex/cal_smatrix_RGF_mex.c) or a ctypes binding binds to reproduce them.
  *
  * Conventions
  *  - Plain C types only; inputs are borrowed for the call and never mutated;
@@ -31,7 +31,8 @@
 extern "C" {
 #endif
 
-#define RGF_ABI_VERSION 2  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate */
+#define RGF_ABI_VERSION 3  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate;
+                              3: rgf_info.max_multiplier */
 #define RGF_ALL (-1) /* selection count meaning "all propagating channels 1..N_prop" */
 
 typedef enum {
@@ -129,9 +130,13 @@ typedef struct {
        stop, 4 q/(1-q) max|S_k - S_(k-1)| with q the ratio of the last two
        residuals (linear convergence, a factor 2 to spare); 0 otherwise */
     double refine_s_err_estimate;
+    /* largest multiplier |(A[:, P] A_PP^-1)_ij| of the slice inverses (the
+       growth the rows outside a diagonal block see; pivoting is confined to
+       the block).  The FP64 paths fail with RGF_ERR_NUMERICAL above 1e6. */
+    double max_multiplier;
 } rgf_info;
 
-/* Lead channels (setup_channels.m:30-82; setup_channels_3D.m:335-370).
+/* Lead channels (setup_channels.m:30-82; setup_channels_3D.m:30-64).
  * Every array has N = transverse size entries (caller-allocated); complex
  * arrays are interleaved with 2N doubles.  Sides share data when
  * epsilon_R == epsilon_L, as in the reference (setup_channels.m:166-167). */

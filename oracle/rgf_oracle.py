@@ -36,7 +36,7 @@ Conventions that differ from a literal transcription (documented, not bugs):
   (the representative MATLAB returns); complex z -> principal branch, negated
   where Im z<0 and (Re z<0 or Re z>1).
 * Bloch BC with n=1 in build_laplacian_1d: spdiags receives diagonal 0 twice
-  (build_laplacian_1d.m:289); MATLAB's sparse() sums duplicates, giving
+  (build_laplacian_1d.m:51); MATLAB's sparse() sums duplicates, giving
   -4 sin^2(ka/2), which is what is implemented.
 """
 from __future__ import annotations
@@ -66,7 +66,7 @@ _BC_ALIASES = {
 def parse_bc(BC) -> tuple[str, Any]:
     """Normalise a BC argument to ('Bloch', ka) or (name, None).
 
-    Follows setup_channels.m:102-115 / build_laplacian_1d.m:271-282: a numeric
+    Follows setup_channels.m:102-115 / build_laplacian_1d.m:31-44: a numeric
     BC means Bloch with ka=BC; 'periodic' means Bloch with ka=0; 'Bloch' as a
     string is an error; other strings are case-insensitive names.
     """
@@ -87,12 +87,12 @@ def parse_bc(BC) -> tuple[str, Any]:
 
 
 def build_laplacian_1d(n: int, BC) -> np.ndarray:
-    """Dense (d/dx)^2 dx^2 as -grad1^H grad1 (build_laplacian_1d.m:284-313)."""
+    """Dense (d/dx)^2 dx^2 as -grad1^H grad1 (build_laplacian_1d.m:46-75)."""
     if not (isinstance(n, (int, np.integer)) and n >= 0):
         raise ValueError("The number of sites n must be a non-negative integer scalar")
     kind, ka = parse_bc(BC)
     if kind == "Bloch":
-        # rows m: df(m-0.5) = f(m) - f(m-1), with f(0) = f(n) exp(-i ka) (:287-289)
+        # rows m: df(m-0.5) = f(m) - f(m-1), with f(0) = f(n) exp(-i ka) (:48-51)
         g = np.zeros((n, n), dtype=complex)
         for m in range(n):
             g[m, m] += 1.0
@@ -100,25 +100,25 @@ def build_laplacian_1d(n: int, BC) -> np.ndarray:
                 g[m, m - 1] += -1.0
         if n >= 1:
             g[0, n - 1] += -cmath.exp(-1j * ka)   # duplicates add, as sparse() does
-    elif kind == "Dirichlet":       # (:290-293), (n+1) x n
+    elif kind == "Dirichlet":       # (:52-55), (n+1) x n
         g = np.zeros((n + 1, n), dtype=complex)
         for m in range(n + 1):
             if m < n:
                 g[m, m] = 1.0
             if m >= 1:
                 g[m, m - 1] = -1.0
-    elif kind == "Neumann":         # (:294-297), (n-1) x n
+    elif kind == "Neumann":         # (:56-59), (n-1) x n
         g = np.zeros((max(n - 1, 0), n), dtype=complex)
         for m in range(n - 1):
             g[m, m + 1] = 1.0
             g[m, m] = -1.0
-    elif kind == "DirichletNeumann":  # (:298-301), n x n
+    elif kind == "DirichletNeumann":  # (:60-63), n x n
         g = np.zeros((n, n), dtype=complex)
         for m in range(n):
             g[m, m] = 1.0
             if m >= 1:
                 g[m, m - 1] = -1.0
-    else:                           # NeumannDirichlet (:302-305), n x n
+    else:                           # NeumannDirichlet (:64-67), n x n
         g = np.zeros((n, n), dtype=complex)
         for m in range(n):
             g[m, m] = -1.0
@@ -147,7 +147,7 @@ def kx_from_z(z: complex) -> complex:
 
 
 def _longitudinal(k0dx2_eps: complex, sin2_t: np.ndarray) -> dict:
-    """setup_longitudinal (setup_channels.m:179-219; setup_channels_3D.m:452-492).
+    """setup_longitudinal (setup_channels.m:179-219; setup_channels_3D.m:146-188).
 
     ``sin2_t`` is sum of sin^2(k_t/2) over the transverse directions, in the
     channel order (column-major in 3D).  Returns 1-based ``ind_prop``.
@@ -230,7 +230,7 @@ def mode_matrix_2d(ny: int, kind: str, kydx: np.ndarray) -> np.ndarray:
 
 
 def setup_channels_3D(nx: int, ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
-    """3D lead channels (setup_channels_3D.m:372-449).
+    """3D lead channels (setup_channels_3D.m:76-140).
 
     ``kxdx``/``kydx`` are nx x ny arrays; channel index a is the column-major
     linear index (x fast).  ``phi`` is the nxy x nxy mode matrix whose column a
@@ -287,7 +287,7 @@ def setup_channels_3D(nx: int, ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> 
 
 
 def mode_matrix_3d(nx, ny, kind, kx1, ky1) -> np.ndarray:
-    """nxy x nxy mode matrix; row (ix + nx*iy), column (a + nx*b) (setup_channels_3D.m:424,429)."""
+    """nxy x nxy mode matrix; row (ix + nx*iy), column (a + nx*b) (setup_channels_3D.m:118,123)."""
     mx = np.arange(1, nx + 1)[:, None]
     my = np.arange(1, ny + 1)[:, None]
     if kind == "Bloch":
@@ -555,16 +555,19 @@ def brute_force_smatrix(slices, A0, k0dx2, phi, chL, chR, epsL, epsR,
     return S
 
 
-def transmission_gradient_fd(syst: dict, pixels, h: float = 1e-6, three_d: bool = False):
-    """Central finite differences of T = sum |t_ij|^2 (in L, out R) w.r.t. real eps pixels."""
+def transmission_gradient_fd(syst: dict, pixels, h: float = 1e-6, three_d: bool = False, part: str = "re"):
+    """Central finite differences of T = sum |t_ij|^2 (in L, out R) w.r.t. the
+    real (part="re") or imaginary (part="im") part of eps pixels."""
     solve = cal_smatrix_RGF_3D if three_d else cal_smatrix_RGF
-    eps0 = np.array(syst["epsilon"], dtype=complex if np.iscomplexobj(syst["epsilon"]) else float)
+    step = {"re": 1.0, "im": 1j}[part]
+    cplx = np.iscomplexobj(syst["epsilon"]) or part == "im"
+    eps0 = np.array(syst["epsilon"], dtype=complex if cplx else float)
     out = []
     for idx in pixels:
         vals = []
         for s in (+1, -1):
             e = eps0.copy()
-            e[idx] = e[idx] + s * h
+            e[idx] = e[idx] + s * h * step
             S, _ = solve({**syst, "epsilon": e}, ["R"], ["L"])
             vals.append(float(np.sum(np.abs(S) ** 2)))
         out.append((vals[0] - vals[1]) / (2 * h))

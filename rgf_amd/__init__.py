@@ -4,7 +4,7 @@ The numerical work lives in ``librgf.so`` (hand-written gfx950 HIP kernels
 behind the C ABI of ``include/rgf.h``); this package mirrors the reference's
 MATLAB functions on top of it.
 """
-from ._lib import RGFError, load_library  # noqa: F401
+from ._lib import RGFError, load_library, trim  # noqa: F401
 from .api import (  # noqa: F401
     build_laplacian_1d,
     cal_smatrix_RGF,
@@ -21,5 +21,5 @@ from .sweep import load_scan, save_scan, scan  # noqa: F401
 __all__ = [
     "cal_smatrix_RGF", "cal_smatrix_RGF_3D", "setup_channels", "setup_channels_3D",
     "build_laplacian_1d", "load_library", "RGFError", "dense_inverse", "dense_gemm",
-    "transmission_gradient", "cal_smatrix_RGF_batched", "scan", "save_scan", "load_scan",
+    "transmission_gradient", "cal_smatrix_RGF_batched", "scan", "save_scan", "load_scan", "trim",
 ]

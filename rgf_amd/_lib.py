@@ -1,6 +1,7 @@
 """ctypes binding of librgf.so (the C ABI declared in include/rgf.h)."""
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import threading
@@ -64,7 +65,7 @@ class Info(C.Structure):
         ("refine_residual", C.c_double), ("seconds_total", C.c_double), ("seconds_setup", C.c_double),
         ("seconds_slices", C.c_double), ("seconds_refine", C.c_double), ("flops_slices", C.c_double),
         ("kernel_seconds", C.c_double), ("kernel_flops", C.c_double), ("kernel_launches", C.c_int64),
-        ("refine_s_err_estimate", C.c_double),
+        ("refine_s_err_estimate", C.c_double), ("max_multiplier", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -108,6 +109,10 @@ SIGNATURES = {
 _lib = None
 _lock = threading.Lock()
 _handles: dict[int, int] = {}
+# A handle's cached device buffers and stream are not safe for concurrent use
+# (ctypes releases the GIL during the call): every call that takes a handle
+# holds that handle's lock, including set_stream / solve / reset sequences.
+_handle_locks: dict[int, threading.RLock] = {}
 
 
 class RGFError(RuntimeError):
@@ -154,7 +159,8 @@ def check(status: int, handle=None) -> None:
 
 
 def handle(device: int = -1) -> int:
-    """The process-wide handle of a device (created on first use)."""
+    """The process-wide handle of a device (created on first use).  Calls that
+    pass it to the library must hold its lock: use ``using(device)``."""
     lib = load_library()
     with _lock:
         if device not in _handles:
@@ -163,7 +169,23 @@ def handle(device: int = -1) -> int:
             if st != RGF_OK:
                 raise RGFError(st, lib.rgf_last_error(None).decode())
             _handles[device] = h.value
+            _handle_locks[h.value] = threading.RLock()
         return _handles[device]
+
+
+@contextlib.contextmanager
+def using(device: int = -1):
+    """The device's handle, held exclusively by this thread for the block."""
+    h = handle(device)
+    with _handle_locks[h]:
+        yield h
+
+
+def trim(device: int = -1) -> None:
+    """Release the device workspace cached by the device's handle (rgf_trim)."""
+    lib = load_library()
+    with using(device) as h:
+        check(lib.rgf_trim(h), h)
 
 
 def dptr(a: np.ndarray):

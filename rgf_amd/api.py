@@ -59,7 +59,7 @@ def _bc_1d(BC) -> tuple[int, float]:
 
 
 def _bc_xy(BC) -> tuple[list[int], list[float]]:
-    """setup_channels_3D.m:387-405: a string for both directions or [kxa, kya]."""
+    """setup_channels_3D.m:81-99: a string for both directions or [kxa, kya]."""
     if isinstance(BC, str):
         low = BC.lower()
         if low == "bloch":
@@ -300,18 +300,19 @@ def _solve(prob: _Prob, out, in_, precision, device, opts, fields: bool = False)
         if device is not None and device >= 0 and device != prob.device:
             raise ValueError(f"syst.epsilon is on device {prob.device}, the solve was asked for device {device}")
         device = prob.device
-    h = L.handle(device)
     S = np.zeros((nout.value, nin.value), dtype=np.complex128, order="F")
     info = L.Info()
     o = _options(precision, **(opts or {}))
-    if fields:
-        F = np.zeros((N, nslab, nin.value), dtype=np.complex128, order="F")
-        st = lib.rgf_solve_fields(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp),
-                                  F.ctypes.data_as(L._dp), C.byref(info))
+    with L.using(device) as h:
+        if fields:
+            F = np.zeros((N, nslab, nin.value), dtype=np.complex128, order="F")
+            st = lib.rgf_solve_fields(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp),
+                                      F.ctypes.data_as(L._dp), C.byref(info))
+            L.check(st, h)
+            return S, info.as_dict(), F
+        st = lib.rgf_solve(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp),
+                           C.byref(info))
         L.check(st, h)
-        return S, info.as_dict(), F
-    st = lib.rgf_solve(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), S.ctypes.data_as(L._dp), C.byref(info))
-    L.check(st, h)
     return S, info.as_dict()
 
 
@@ -386,8 +387,8 @@ def cal_smatrix_RGF_batched(items, *, precision: str = "fp64", device: int = -1,
     outs = (L._dp * n)(*[(S if S.size else dummy).ctypes.data_as(L._dp) for S in Ss])
     infos = (L.Info * n)()
     o = _options(precision, **(options or {}))
-    h = L.handle(device)
-    L.check(lib.rgf_solve_batched(h, n, P, Sel, C.byref(o), outs, infos), h)
+    with L.using(device) as h:
+        L.check(lib.rgf_solve_batched(h, n, P, Sel, C.byref(o), outs, infos), h)
     res = [S if S.size else np.zeros(S.shape) for S in Ss]
     return (res, [i.as_dict() for i in infos]) if return_info else res
 
@@ -457,7 +458,7 @@ def setup_channels(ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
 
 
 def setup_channels_3D(nx: int, ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
-    """channels = setup_channels_3D(nx, ny, BC, k0dx, epsilon_L, epsilon_R) (setup_channels_3D.m:1-449)."""
+    """channels = setup_channels_3D(nx, ny, BC, k0dx, epsilon_L, epsilon_R) (setup_channels_3D.m:1-188)."""
     for n, lab in ((nx, "nx"), (ny, "ny")):
         if not (isinstance(n, (int, np.integer)) and n > 0):
             raise ValueError(f"{lab} must be a positive integer scalar")
@@ -513,9 +514,9 @@ def dense_inverse(A: np.ndarray, precision: str = "fp64", block_size: int = 0, d
     complex_3m: the FP32 kernels with three real products per complex product."""
     lib = L.load_library()
     M = np.asfortranarray(A, dtype=np.complex128).copy(order="F")
-    h = L.handle(device)
     prec = _PRECISIONS[precision] | (RGF_DENSE_3M if complex_3m else 0)
-    L.check(lib.rgf_dense_inverse(h, prec, M.shape[0], M.ctypes.data_as(L._dp), block_size), h)
+    with L.using(device) as h:
+        L.check(lib.rgf_dense_inverse(h, prec, M.shape[0], M.ctypes.data_as(L._dp), block_size), h)
     return M
 
 
@@ -524,9 +525,9 @@ def dense_gemm(A: np.ndarray, B: np.ndarray, precision: str = "fp64", device: in
     A = np.asfortranarray(A, dtype=np.complex128)
     B = np.asfortranarray(B, dtype=np.complex128)
     Cm = np.zeros((A.shape[0], B.shape[1]), dtype=np.complex128, order="F")
-    h = L.handle(device)
-    L.check(lib.rgf_dense_gemm(h, _PRECISIONS[precision], A.shape[0], B.shape[1], A.shape[1],
-                               A.ctypes.data_as(L._dp), B.ctypes.data_as(L._dp), Cm.ctypes.data_as(L._dp)), h)
+    with L.using(device) as h:
+        L.check(lib.rgf_dense_gemm(h, _PRECISIONS[precision], A.shape[0], B.shape[1], A.shape[1],
+                                   A.ctypes.data_as(L._dp), B.ctypes.data_as(L._dp), Cm.ctypes.data_as(L._dp)), h)
     return Cm
 
 
@@ -551,11 +552,11 @@ def transmission_gradient(syst: dict, out=("R",), in_=("L",), *, three_d: bool =
     gim = np.zeros(n)
     T = C.c_double()
     info = L.Info()
-    h = L.handle(device)
     o = _options("fp64")
-    st = lib.rgf_transmission_gradient(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), int(checkpoint_every),
-                                       C.byref(T), L.dptr(gre), L.dptr(gim), C.byref(info))
-    L.check(st, h)
+    with L.using(device) as h:
+        st = lib.rgf_transmission_gradient(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), int(checkpoint_every),
+                                           C.byref(T), L.dptr(gre), L.dptr(gim), C.byref(info))
+        L.check(st, h)
     gre = gre.reshape(shape, order="F")
     gim = gim.reshape(shape, order="F")
     if return_info:

@@ -32,16 +32,16 @@ class TransmissionFunction(torch.autograd.Function):
         gim = np.zeros(n)
         T = C.c_double()
         dev = eps.device.index if eps.is_cuda else -1
-        h = L.handle(dev if dev is not None else -1)
-        if eps.is_cuda:
-            L.check(lib.rgf_set_stream(h, C.c_void_p(torch.cuda.current_stream(eps.device).cuda_stream)), h)
-        try:
-            st = lib.rgf_transmission_gradient(h, C.byref(prob.p), C.byref(sel.s), C.byref(_options("fp64")),
-                                               int(checkpoint_every), C.byref(T), L.dptr(gre), L.dptr(gim), None)
-        finally:
+        with L.using(dev if dev is not None else -1) as h:
             if eps.is_cuda:
-                lib.rgf_set_stream(h, None)
-        L.check(st, h)
+                L.check(lib.rgf_set_stream(h, C.c_void_p(torch.cuda.current_stream(eps.device).cuda_stream)), h)
+            try:
+                st = lib.rgf_transmission_gradient(h, C.byref(prob.p), C.byref(sel.s), C.byref(_options("fp64")),
+                                                   int(checkpoint_every), C.byref(T), L.dptr(gre), L.dptr(gim), None)
+            finally:
+                if eps.is_cuda:
+                    lib.rgf_set_stream(h, None)
+            L.check(st, h)
         g = gre.reshape(shape, order="F")
         if eps.is_complex():
             g = g + 1j * gim.reshape(shape, order="F")

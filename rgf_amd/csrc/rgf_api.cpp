@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <atomic>
@@ -80,6 +81,16 @@ T* dbuf(rgf_context* c, const std::string& name, size_t count) {
         b.bytes = need;
     }
     return (T*)b.p;
+}
+
+// bytes the named buffers already hold (dbuf reuses or grows them in place)
+size_t held_bytes(rgf_context* c, std::initializer_list<const char*> names) {
+    size_t t = 0;
+    for (const char* n : names) {
+        auto it = c->bufs.find(n);
+        if (it != c->bufs.end()) t += it->second.bytes;
+    }
+    return t;
 }
 
 void release(rgf_context* c, const std::string& name) {
@@ -138,14 +149,37 @@ std::vector<cd> download(rgf_context* c, CMat<double> d, int64_t rows, int64_t c
     return h;
 }
 
-void check_status(rgf_context* c) {
-    int st = 0;
-    RGF_HIP(hipMemcpyAsync(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+// Largest multiplier |(A[:, P] A_PP^-1)_ij| the slice inverses may apply to
+// the rows outside a diagonal block before the result is reported as
+// inaccurate: pivoting is confined to the block, so an ill-conditioned block
+// (in a well-conditioned matrix) shows up as large multipliers, which
+// amplify the rounding of every later update by about that factor.
+double multiplier_limit(bool fp64) { return fp64 ? 1e6 : 1e3; }
+
+// Status of the slice inverses since the last reset_status: a zero or
+// non-finite pivot is an error; so is a multiplier above multiplier_limit
+// when `growth` (the FP64 path and the dense inverse; the mixed path's FP64
+// refinement measures the accuracy it reaches and does not need it).
+double check_status(rgf_context* c, bool growth = true, bool fp64 = true) {
+    int st[2] = {0, 0};
+    RGF_HIP(hipMemcpyAsync(st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
     RGF_HIP(hipStreamSynchronize(c->stream));
-    if (st) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+    if (st[0]) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+    float m2;
+    memcpy(&m2, &st[1], sizeof m2);
+    const double m = std::sqrt((double)m2);
+    if (growth && !(m <= multiplier_limit(fp64))) {
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "ill-conditioned diagonal block in a slice inverse: multiplier %.3g above %.0e (pivoting is "
+                 "confined to the diagonal block, so the inverse would be inaccurate)",
+                 m, multiplier_limit(fp64));
+        throw Error(RGF_ERR_NUMERICAL, buf);
+    }
+    return m;
 }
 
-void reset_status(rgf_context* c) { RGF_HIP(hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream)); }
+void reset_status(rgf_context* c) { RGF_HIP(hipMemsetAsync(c->d_status, 0, 2 * sizeof(int), c->stream)); }
 
 // ---------------------------------------------------------------- problem on the device
 // Everything in sweep order: "L" is the lead the slice loop starts from.
@@ -170,6 +204,7 @@ struct Work {
     int refine_iters = 0;
     double refine_residual = 0;
     double refine_s_err = 0;  // estimated S error at the stop (refine_s_tol)
+    double max_multiplier = 0;  // largest slice-inverse multiplier
     double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
 };
 
@@ -373,7 +408,7 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     w.t_slices = now() - t0;
     const double Nd = (double)N;
     w.flops = (double)(L + 1) * (8 * Nd * Nd * Nd + 8 * Nd * Nd * (pi + po) + 8.0 * po * Nd * pi);
-    check_status(c);
+    w.max_multiplier = check_status(c);
     const int64_t qi = w.inR.size(), qo = w.outR.size();
     if (po && pi) w.TLL = download(c, Z, po, pi);
     if (qo && pi) {
@@ -410,7 +445,10 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     size_t freeb = 0, totalb = 0;
     RGF_HIP(hipMemGetInfo(&freeb, &totalb));
     const double need = (double)S * N * N * 8 + (double)S * N * m * (16 + 16 + 8) + (double)N * N * 8;
-    if (need > 0.92 * (double)freeb)
+    // buffers kept from an earlier solve on this handle are reused (or freed
+    // and grown), so what they hold counts as available
+    const double avail = (double)freeb + (double)held_bytes(c, {"factors", "psi", "res", "delta", "G0R32"});
+    if (need > 0.92 * avail)
         throw Error(RGF_ERR_OUT_OF_MEMORY, "mixed-precision factor store does not fit in device memory");
     float* F = dbuf<float>(c, "factors", (size_t)S * N * N * 2);
     auto Fn = [&](int64_t n) { return CMat<float>(F + (size_t)n * N * N * 2, F + (size_t)n * N * N * 2 + (size_t)N * N, N); };
@@ -435,7 +473,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     }
     RGF_HIP(hipStreamSynchronize(s));
     w.t_slices = now() - t0;
-    check_status(c);
+    w.max_multiplier = check_status(c, false, false);
     const double Nd = (double)N;
     w.flops = (double)(L + 1) * 8 * Nd * Nd * Nd * (use3m ? 0.75 : 1.0);
     release(c, "G0R32");
@@ -712,7 +750,7 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
     }
     RGF_HIP(hipStreamSynchronize(s));
     w.t_slices = now() - t0;
-    check_status(c);
+    w.max_multiplier = check_status(c);
     const double Nd = (double)N;
     const int64_t recomputed = L - (L / C);
     w.flops = (double)(L + 1 + recomputed) * (8 * Nd * Nd * Nd + 8 * Nd * Nd * m) + (double)L * 16 * Nd * Nd * m;
@@ -767,6 +805,7 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
         info->seconds_setup = w.t_setup;
         info->seconds_slices = w.t_slices;
         info->flops_slices = w.flops;
+        info->max_multiplier = w.max_multiplier;
         if (timer) {
             info->kernel_seconds = timer->seconds();
             info->kernel_flops = timer->flops;
@@ -870,6 +909,7 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
         info->refine_iters = w.refine_iters;
         info->refine_residual = w.refine_residual;
         info->refine_s_err_estimate = w.refine_s_err;
+        info->max_multiplier = w.max_multiplier;
         info->seconds_setup = w.t_setup;
         info->seconds_slices = w.t_slices;
         info->seconds_refine = w.t_refine;
@@ -1195,7 +1235,7 @@ rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double*
             invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status, use3m, nullptr, &h->ov);
             convert<double, float>(h->stream, n, n, M32, M, 1.0);
         }
-        check_status(h);
+        check_status(h, true, precision == RGF_PRECISION_FP64);
         download_colmajor(h, M, n, n, A);
     });
 }

@@ -165,7 +165,7 @@ Channels setup_channels(const rgf_problem* p) {
     std::vector<double> sin2(ch.N);
     for (int64_t a2 = 0; a2 < ch.n2; ++a2)
         for (int64_t a1 = 0; a1 < ch.n1; ++a1) {
-            const int64_t a = a1 + ch.n1 * a2;  // column-major, x fast (setup_channels_3D.m:421-422)
+            const int64_t a = a1 + ch.n1 * a2;  // column-major, x fast (setup_channels_3D.m:114-118)
             ch.kt1[a] = k1[a1];
             ch.kt2[a] = ch.ndim == 3 ? k2[a2] : 0.0;
             const double s1 = std::sin(k1[a1] / 2), s2 = ch.ndim == 3 ? std::sin(k2[a2] / 2) : 0.0;
@@ -187,7 +187,7 @@ Channels setup_channels(const rgf_problem* p) {
     return ch;
 }
 
-// build_laplacian_1d.m:284-313 as the sparse product -grad1^H grad1; entries
+// build_laplacian_1d.m:46-75 as the sparse product -grad1^H grad1; entries
 // with equal (row, col) accumulate, as MATLAB's sparse() does (this fixes the
 // Bloch n = 1 case to -4 sin^2(ka/2)).
 std::map<std::pair<int64_t, int64_t>, cd> laplacian_1d(int64_t n, int bc, double ka) {

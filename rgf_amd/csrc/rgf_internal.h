@@ -94,8 +94,11 @@ void cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T> B,
 
 // In-place inverse of the N x N matrix A by blocked Gauss-Jordan elimination
 // with partial pivoting inside each diagonal block.  ws must hold
-// inverse_workspace_elems<T>(N, b) elements of T.  *dev_status != 0 after the
-// call signals a zero or non-finite pivot.
+// inverse_workspace_elems<T>(N, b) elements of T.  dev_status points to two
+// words: dev_status[0] != 0 after the call signals a zero or non-finite
+// pivot; dev_status[1] receives (atomic max) the largest |multiplier|^2 of
+// the column panels A[:, P] A_PP^-1 as non-negative float bits.  Both are
+// accumulated, not cleared.
 template <typename T>
 size_t inverse_workspace_elems(int64_t N, int b);
 template <typename T>

@@ -120,7 +120,17 @@ struct GemmArgs {
     int nz, kc;       // split-K: nz chunks of kc along K, chunk z writes C + z * zstride
     int64_t zstride;
     int tiles_n, tiles;  // tile grid of this problem (set at launch)
+    // optional: max |C_ij|^2 over the stored entries, as the bits of a
+    // non-negative float (which order like unsigned integers), atomicMax'ed
+    unsigned* amax;
 };
+
+// max |x|^2 of a wavefront's values into *amax (non-negative float bits)
+__device__ __forceinline__ void wave_amax(unsigned* amax, float m) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(isnan(m) ? INFINITY : m));
+}
 
 // A 1D grid covers the tiles of g0 followed by those of g1: two independent
 // problems sharing the tile configuration run as one launch.
@@ -383,6 +393,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
     }
 
     // epilogue
+    float mloc = 0.f;
 #pragma unroll
     for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -416,6 +427,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
                     }
                     Cre[o] = pr;
                     Cim[o] = pi;
+                    mloc = fmaxf(mloc, (float)(pr * pr + pi * pi));
                     if (gi >= g.y_rlo && gi < g.y_rhi && gj >= g.y_clo && gj < g.y_chi) {
                         const int64_t oy = (int64_t)(gi - g.y_rlo) * g.ldy + (gj - g.y_clo);
                         g.Yre[oy] = pr;
@@ -423,6 +435,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
                     }
                 }
             }
+    if (g.amax) wave_amax(g.amax, mloc);
 }
 
 // Streamlined tile kernel for alpha = +-1, beta in {0, 1}, K a multiple of
@@ -744,6 +757,8 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     }
 
     // epilogue (the accumulators started from C when beta = 1)
+    unsigned* const amax = g.amax;
+    float mloc = 0.f;
     if constexpr (TR) {
         const bool vc = (ldc % 4) == 0 && (((uintptr_t)Cre | (uintptr_t)Cim) & 15) == 0;
 #pragma unroll
@@ -781,6 +796,10 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                         *(VT*)(Cre + o + u) = xr;
                         *(VT*)(Cim + o + u) = xi;
                     }
+                    if (amax) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) mloc = fmaxf(mloc, (float)(vr[r] * vr[r] + vi[r] * vi[r]));
+                    }
                 } else {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
@@ -788,6 +807,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                         if (c < N && !(c >= g.sclo && c < g.schi) && !(rblk && c >= g.sblo && c < g.sbhi)) {
                             Cre[o + r] = vr[r];
                             Cim[o + r] = vi[r];
+                            if (amax) mloc = fmaxf(mloc, (float)(vr[r] * vr[r] + vi[r] * vi[r]));
                         }
                     }
                 }
@@ -804,6 +824,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     }
                 }
             }
+        if (amax) wave_amax(amax, mloc);
         return;
     }
 #pragma unroll
@@ -821,6 +842,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     if constexpr (ACC3) vr -= ii[a][b][r];
                     Cre[(int64_t)gi * ldc + gj] = vr;
                     Cim[(int64_t)gi * ldc + gj] = ri[a][b][r];
+                    if (amax) mloc = fmaxf(mloc, (float)(vr * vr + ri[a][b][r] * ri[a][b][r]));
                     if (gi >= g.y_rlo && gi < g.y_rhi && gj >= g.y_clo && gj < g.y_chi) {
                         const int64_t oy = (int64_t)(gi - g.y_rlo) * g.ldy + (gj - g.y_clo);
                         g.Yre[oy] = vr;
@@ -829,6 +851,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                 }
             }
         }
+    if (amax) wave_amax(amax, mloc);
 }
 
 template <typename T>
@@ -883,6 +906,7 @@ static GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T>
     g.zstride = 0;
     g.vec_ok = (A.ld % VW == 0) && (B.ld % VW == 0) && ((uintptr_t)A.re % 16 == 0) && ((uintptr_t)A.im % 16 == 0) &&
                ((uintptr_t)B.re % 16 == 0) && ((uintptr_t)B.im % 16 == 0);
+    g.amax = nullptr;
     return g;
 }
 
@@ -1895,9 +1919,14 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
     CMat<T> D(ws + 2 * set, ws + 2 * set + (size_t)b * b, b);
     const int64_t nblk = (N + b - 1) / b;
     // R = D A_P: and C = -A_:P D in one launch (D from the diagonal block)
+    // the largest multiplier |(A[:, P] D)_ij| of every step goes to the word
+    // after the status flags (pivoting is confined to the diagonal block,
+    // so this is the growth the rows outside the block see)
+    unsigned* const amax = (unsigned*)(dev_status + 1);
     auto panel_products = [&](hipStream_t st, int64_t p0, int nb, int q) {
         const GemmArgs<T> gr = gemm_args<T>(nb, N, nb, D, A.offset(p0, 0), R[q], cd(1, 0), 0, Skip{});
-        const GemmArgs<T> gc = gemm_args<T>(N, nb, nb, A.offset(0, p0), D, C[q], cd(-1, 0), 0, Skip{});
+        GemmArgs<T> gc = gemm_args<T>(N, nb, nb, A.offset(0, p0), D, C[q], cd(-1, 0), 0, Skip{});
+        gc.amax = amax;
         cgemm_pair<T>(st, gr, gc, use3m);
     };
     auto panels = [&](hipStream_t st, int64_t p0, int nb, int q) {
@@ -1973,6 +2002,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         const int nb = nb_of(q), nb1 = nb_of(q + 1);
         GemmArgs<T> gr = gemm_args<T>(nb, N, nb, D, Rold, A.offset(p0, 0), cd(1, 0), 0, Skip{0, 0, p0, p1});
         GemmArgs<T> gc = gemm_args<T>(N, nb, nb, Aold[q & 1], D, A.offset(0, p0), cd(-1, 0), 0, Skip{p0, p1, 0, 0});
+        gc.amax = amax;
         if (nb1 > 0) {
             const CMat<T> An = Aold[(q + 1) & 1].offset(p0, 0), Rp = Rold.offset(0, p0);
             gr.Yre = An.re;

@@ -8,6 +8,7 @@ Inputs (eps arrays, scalars, selections) are stored with the outputs so the
 GPU tests consume identical data.
 
 Run from the repository root:  python tests/golden/make_golden.py
+(``--full`` writes rgf_golden_full.npz, the full-transverse-size cases).
 """
 import os
 import sys
@@ -95,5 +96,50 @@ def main():
     print(f"wrote {len(names)} cases to {OUT}")
 
 
+OUT_FULL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rgf_golden_full.npz")
+
+
+def main_full():
+    """Oracle S matrices at the benchmark workloads' full transverse sizes,
+    on a few slices each (the oracle's general scheme costs 32 N^3 per slice):
+      * 2d_n1024: the 2D large-slab workload (N = 1024, Dirichlet, eps_L = 1,
+        eps_R = 2.25, lambda = 10 px, eps = 1 + 1.25 Bernoulli(0.3) + 0.01i),
+        its first 32 slices, full S (528 x 528);
+      * 3d_n4096: the 3D workload (64 x 64 transverse, 40 spheres, Bloch
+        phases, lambda = 12 px), its first 6 slices, full S.
+    These sizes are beyond the brute-force check of main(); the oracle's
+    recursion is the same code that the small cases pin."""
+    sys.path.insert(0, ROOT)
+    from rgf_amd.workloads import slab_2d_large, spheres_3d
+    store = {}
+    syst, out, inn = slab_2d_large(nx=32)
+    S, _ = o.cal_smatrix_RGF(syst, out, inn)
+    eps = syst["epsilon"]
+    # eps takes two values: store the Bernoulli mask (exact reconstruction)
+    mask = np.real(eps) > 1.5
+    assert np.array_equal(eps, 1 + 1.25 * mask + 0.01j)
+    store["2d_n1024/mask"] = np.packbits(mask, axis=None)
+    store["2d_n1024/shape"] = np.array(eps.shape)
+    store["2d_n1024/params"] = np.array([syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"]], dtype=complex)
+    store["2d_n1024/S"] = S
+    print("2d_n1024", S.shape, flush=True)
+    syst, out, inn = spheres_3d()
+    eps = syst["epsilon"][:, :, :6].copy()
+    S, _ = o.cal_smatrix_RGF_3D(dict(syst, epsilon=eps), out, inn)
+    mask = eps > 1.5
+    assert np.array_equal(eps, np.where(mask, 2.1, 1.0))
+    store["3d_n4096/mask"] = np.packbits(mask, axis=None)
+    store["3d_n4096/shape"] = np.array(eps.shape)
+    store["3d_n4096/params"] = np.array([syst["k0dx"], syst["epsilon_L"], syst["epsilon_R"]], dtype=complex)
+    store["3d_n4096/xyBC"] = np.array(syst["xyBC"], dtype=float)
+    store["3d_n4096/S"] = S
+    print("3d_n4096", S.shape, flush=True)
+    np.savez_compressed(OUT_FULL, **store)
+    print(f"wrote {OUT_FULL}")
+
+
 if __name__ == "__main__":
-    main()
+    if "--full" in sys.argv:
+        main_full()
+    else:
+        main()

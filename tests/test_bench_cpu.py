@@ -37,11 +37,14 @@ def test_roofline_record():
     assert abs(r["frac"] - r["achieved"] / 157.3) < 1e-3
     assert abs(r["avg_us"] - 150.0) < 1e-6
     json.dumps(r)
-    # three real products: the library counts 6 flops per multiply-add, the record 8
+    # three real products: the library counts 6 flops per multiply-add and the
+    # record keeps them (frac from executed flops); the 8-flop rate is separate
     info3 = dict(info, kernel_flops=info["kernel_flops"] * 0.75)
     r3 = bench.roofline(info3, "fp32", use3m=True)
-    assert abs(r3["achieved"] - r["achieved"]) < 0.01
-    assert abs(r3["executed_tflops"] - 0.75 * r["achieved"]) < 0.02
+    assert abs(r3["achieved"] - 0.75 * r["achieved"]) < 0.02
+    assert abs(r3["frac"] - r3["achieved"] / 157.3) < 1e-3
+    assert abs(r3["effective_tflops_4m"] - r["achieved"]) < 0.02
+    assert r3["flops_per_launch"] == 0.75 * flops
     assert bench.roofline({"kernel_launches": 0}, "fp32") is None
 
 

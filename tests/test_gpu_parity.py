@@ -104,6 +104,43 @@ def test_slice_inverse_fallback(precision, n, b, tol, where):
     assert np.abs(X - ref).max() / np.abs(ref).max() < tol
 
 
+def _second_block_nearly_singular(rng, n=256, b=64, scale=1e-9):
+    """A well-conditioned matrix whose Schur complement block at rows and
+    columns [b, 2b) is nearly singular: a block-row swap of a diagonally
+    dominant matrix whose rows [2b, 3b) are tiny in the columns [0, 2b)."""
+    B = 3 * np.eye(n) + 0.2 * _crand(rng, n, n) / np.sqrt(n)
+    B[2 * b:3 * b, :2 * b] *= scale
+    P = np.arange(n)
+    P[b:2 * b], P[2 * b:3 * b] = np.arange(2 * b, 3 * b), np.arange(b, 2 * b)
+    return B[P]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_slice_inverse_ill_conditioned_block_fails_loudly(precision):
+    # Pivoting is confined to the diagonal block: a block that needs
+    # pivoting and is ill-conditioned must be reported, not inverted silently
+    # with the growth of its multipliers (RGF_ERR_NUMERICAL).
+    rng = np.random.default_rng(11)
+    A = _second_block_nearly_singular(rng)
+    assert np.linalg.cond(A) < 10
+    with pytest.raises(rgf_amd.RGFError, match="ill-conditioned diagonal block"):
+        rgf_amd.dense_inverse(A, precision, 64)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
+def test_high_index_slab_vs_oracle(precision, tol):
+    # eps up to 12 at 6 px per vacuum wavelength: k0dx^2 eps runs through the
+    # band edge 4 (slice matrices far from diagonal dominance)
+    rng = np.random.default_rng(12)
+    eps = 1 + 11 * rng.random((256, 40))
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=2.0, k0dx=2 * np.pi / 6, yBC="periodic")
+    S, _ = rgf_amd.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"], precision=precision)
+    Sref, _ = o.cal_smatrix_RGF(syst, ["L", "R"], ["L", "R"])
+    err = float(np.abs(S - Sref).max())
+    print(f"high-index slab {precision}: {err:.2e}", flush=True)
+    assert err < tol
+
+
 def _cells(s):
     return [c for c in s]
 

@@ -128,7 +128,7 @@ def test_channels_3d_column_major_and_phi():
     A0 = o.slice_operator_3d(nx, ny, [0.3, -1.1])
     lam = 2 + 4 * (np.sin(ch["kxdx"] / 2) ** 2 + np.sin(ch["kydx"] / 2) ** 2).ravel(order="F")
     assert np.allclose(A0 @ phi, phi * lam[None, :], atol=1e-12)
-    # column a = vec(fun_phi(kx(a), ky(a))) with x fast (setup_channels_3D.m:424)
+    # column a = vec(fun_phi(kx(a), ky(a))) with x fast (setup_channels_3D.m:118)
     a = 7
     ix, iy = a % nx, a // nx
     f = np.exp(1j * np.arange(1, nx + 1)[:, None] * ch["kxdx"][ix, iy]) * \
@@ -438,3 +438,75 @@ def test_high_precision_tiny():
                     acc += phi[i, ar] * Hi[sr * ny + i, sc * ny + j] * phi[j, ac]
             val = -2j * mp.sqrt(mp.sin(kr[ar])) * mp.sqrt(mp.sin(kc[ac])) * acc - (1 if r == c else 0)
             assert abs(complex(val) - S[r, c]) < 1e-13
+
+
+# ----------------------------------------------------------------------------- adjoint gradient
+# transmission_gradient takes the forward fields and the adjoint rows from one
+# sparse LU of the whole system; the finite differences below difference the
+# S matrix of the slice recursion (cal_smatrix_RGF, itself pinned by the
+# brute-force tests above).  Both parts are checked: a wrong sign or a dropped
+# conjugate in dT/dIm eps, or a transposed Psi/Phi index, fails here.
+@pytest.mark.parametrize("bc", BCS_2D)
+@pytest.mark.parametrize("cplx_k", [False, True])
+def test_transmission_gradient_vs_central_differences_2d(bc, cplx_k):
+    rng = np.random.default_rng(17 + BCS_2D.index(bc) + 100 * cplx_k)
+    ny, nx = 7, 5
+    eps = _rand_eps(rng, (ny, nx), cplx=True)
+    k0dx = 1.3 * (1 + 0.01j) if cplx_k else 1.3
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.4, k0dx=k0dx, yBC=bc)
+    T, gre, gim = o.transmission_gradient(syst)
+    S, _ = o.cal_smatrix_RGF(syst, ["R"], ["L"])
+    assert abs(T - float(np.sum(np.abs(S) ** 2))) < 1e-12 * max(1.0, T)
+    pixels = [(i, j) for i in range(ny) for j in range(nx)]
+    fd_re = o.transmission_gradient_fd(syst, pixels, h=1e-5, part="re")
+    fd_im = o.transmission_gradient_fd(syst, pixels, h=1e-5, part="im")
+    scale = max(np.abs(fd_re).max(), np.abs(fd_im).max())
+    assert scale > 1e-3
+    assert np.abs(gre[tuple(np.array(pixels).T)] - fd_re).max() < 1e-7 * scale
+    assert np.abs(gim[tuple(np.array(pixels).T)] - fd_im).max() < 1e-7 * scale
+
+
+def test_transmission_gradient_vs_central_differences_real_eps():
+    # real eps: dT/dRe eps from FD of a real perturbation, dT/dIm eps from a
+    # perturbation that makes the slab lossy
+    rng = np.random.default_rng(5)
+    eps = 1 + 2 * rng.random((6, 4))
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.0, k0dx=1.1, yBC="periodic")
+    T, gre, gim = o.transmission_gradient(syst)
+    pixels = [(i, j) for i in range(6) for j in range(4)]
+    fd_re = o.transmission_gradient_fd(syst, pixels, h=1e-5, part="re")
+    fd_im = o.transmission_gradient_fd(syst, pixels, h=1e-5, part="im")
+    idx = tuple(np.array(pixels).T)
+    scale = max(np.abs(fd_re).max(), np.abs(fd_im).max())
+    assert np.abs(gre[idx] - fd_re).max() < 1e-7 * scale
+    assert np.abs(gim[idx] - fd_im).max() < 1e-7 * scale
+    # a passive real slab: T is reduced by any absorption, so dT/dIm eps <= 0
+    assert np.all(gim <= 1e-12)
+
+
+@pytest.mark.parametrize("bc", [[0.4, 2.1], "Dirichlet"])
+def test_transmission_gradient_vs_central_differences_3d(bc):
+    rng = np.random.default_rng(23)
+    nx, ny, nz = 3, 4, 3
+    eps = _rand_eps(rng, (nx, ny, nz), cplx=True)
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.2, k0dx=1.2 + 0.005j, xyBC=bc)
+    T, gre, gim = o.transmission_gradient(syst, three_d=True)
+    pixels = [(i, j, k) for i in range(nx) for j in range(ny) for k in range(nz)]
+    fd_re = o.transmission_gradient_fd(syst, pixels, h=1e-5, three_d=True, part="re")
+    fd_im = o.transmission_gradient_fd(syst, pixels, h=1e-5, three_d=True, part="im")
+    idx = tuple(np.array(pixels).T)
+    scale = max(np.abs(fd_re).max(), np.abs(fd_im).max())
+    assert np.abs(gre[idx] - fd_re).max() < 1e-7 * scale
+    assert np.abs(gim[idx] - fd_im).max() < 1e-7 * scale
+
+
+def test_transmission_gradient_fd_catches_planted_sign_error():
+    # the FD check above must fail for a gradient with a flipped Im part
+    rng = np.random.default_rng(9)
+    eps = _rand_eps(rng, (5, 3), cplx=True)
+    syst = dict(epsilon=eps, epsilon_L=1.0, epsilon_R=1.0, k0dx=1.2, yBC="Dirichlet")
+    _, _, gim = o.transmission_gradient(syst)
+    pixels = [(i, j) for i in range(5) for j in range(3)]
+    fd_im = o.transmission_gradient_fd(syst, pixels, h=1e-5, part="im")
+    idx = tuple(np.array(pixels).T)
+    assert np.abs(-gim[idx] - fd_im).max() > 1e-3 * np.abs(fd_im).max()
