@@ -102,9 +102,16 @@ typedef struct {
                                  the kernel_* fields)                                  */
     int32_t single_stream;    /* 1: no look-ahead second stream in the slice inverse
                                  (same kernels, one stream; for counter collection)   */
-    int32_t batch_concurrency;/* rgf_solve_batched: solves in flight at once, each on
-                                 its own streams and workspace; 0 = 8                 */
-    int32_t reserved[3];
+    int32_t batch_concurrency;/* rgf_solve_batched, problems outside the lockstep
+                                 engine: solves in flight at once, each on its own
+                                 streams and workspace; 0 = 8                        */
+    int32_t batch_engine;     /* rgf_solve_batched: 0 = mixed-precision problems up to
+                                 1024 transverse sites run in the lockstep engine, its
+                                 block steps replayed as hipGraphs; 1 = the same with
+                                 plain launches; 2 = every problem on worker contexts */
+    int32_t batch_graph_rounds;/* lockstep engine: block-step rounds per captured
+                                 graph (0 = 16)                                      */
+    int32_t reserved[1];
     double refine_s_tol;      /* FP32_REFINED, > 0: also stop once the estimated max
                                  error of the S entries is below this (see
                                  rgf_info.refine_s_err_estimate); 0 = off (default)  */
@@ -202,11 +209,16 @@ rgf_status rgf_solve_fields(rgf_handle h, const rgf_problem* p, const rgf_select
                             const rgf_options* opt, double* S_out, double* fields_out,
                             rgf_info* info);
 
-/* Many independent solves (frequencies, Bloch phases, widths) in one call:
- * up to opt->batch_concurrency solves run at once, each on its own worker
- * context (streams and device workspace, kept by the handle for later calls).
- * Results are identical to rgf_solve of each problem.  On failure the status
- * of the first failing problem is returned and rgf_last_error names it. */
+/* Many independent solves (frequencies, Bloch phases, widths) in one call.
+ * Mixed-precision problems with up to 1024 transverse sites (batch_engine 0
+ * or 1) run together in the lockstep engine: every launch advances all of
+ * them by one 64-wide block step of their slice inverses (or one slice of
+ * their refinement sweeps), so small slices fill the GPU.  The others run
+ * up to opt->batch_concurrency at once, each on its own worker context
+ * (streams and device workspace, kept by the handle for later calls).
+ * Each S equals rgf_solve's to its precision's tolerance.  On failure the
+ * status of the first failing problem is returned and rgf_last_error names
+ * it. */
 rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p,
                              const rgf_selection* sel, const rgf_options* opt,
                              double* const* S_out, rgf_info* info);

@@ -52,7 +52,8 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("max_refine_iter", C.c_int32), ("refine_tol", C.c_double),
         ("block_size", C.c_int32), ("complex_3m", C.c_int32), ("profile", C.c_int32),
-        ("single_stream", C.c_int32), ("batch_concurrency", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("single_stream", C.c_int32), ("batch_concurrency", C.c_int32), ("batch_engine", C.c_int32),
+        ("batch_graph_rounds", C.c_int32), ("reserved", C.c_int32 * 1),
         ("refine_s_tol", C.c_double),
     ]
 

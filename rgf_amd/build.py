@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "librgf.so")
 DRIVER = os.path.join(HERE, "rgf_inverse_driver")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["rgf_kernels.hip", "rgf_channels.cpp", "rgf_api.cpp"]
+SOURCES = ["rgf_kernels.hip", "rgf_channels.cpp", "rgf_api.cpp", "rgf_batch.cpp"]
 MEX_GATEWAY = os.path.join(HERE, "mex", "cal_smatrix_RGF_mex.c")
 MEX_HARNESS = os.path.join(os.path.dirname(HERE), "tests", "mex_harness")
 

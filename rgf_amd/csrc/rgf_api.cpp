@@ -35,53 +35,20 @@
 #include <thread>
 #include <string>
 
-#include "rgf_host.h"
+#include "rgf_context.h"
 
 using namespace rgf;
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-struct rgf_context {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    Overlap ov;  // look-ahead stream and events of the slice inverse
-    std::string err;
-    std::map<std::string, DevBuf> bufs;
-    int* d_status = nullptr;
-    double* d_scratch = nullptr;
-    std::vector<rgf_context*> workers;  // worker contexts of rgf_solve_batched
-    std::vector<hipEvent_t> timer_events;  // kept across solves (options.profile)
-};
 
 static std::string g_last_error;
 static std::mutex g_err_mu;
 
-namespace {
+namespace rgf {
 
 double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-template <typename T>
-T* dbuf(rgf_context* c, const std::string& name, size_t count) {
-    DevBuf& b = c->bufs[name];
-    const size_t need = std::max<size_t>(count, 1) * sizeof(T);
-    if (b.bytes < need) {
-        if (b.p) {
-            RGF_HIP(hipStreamSynchronize(c->stream));
-            RGF_HIP(hipFree(b.p));
-            b.p = nullptr;
-            b.bytes = 0;
-        }
-        RGF_HIP(hipMalloc(&b.p, need));
-        b.bytes = need;
-    }
-    return (T*)b.p;
-}
 
 // bytes the named buffers already hold (dbuf reuses or grows them in place)
 size_t held_bytes(rgf_context* c, std::initializer_list<const char*> names) {
@@ -103,12 +70,6 @@ void release(rgf_context* c, const std::string& name) {
     c->bufs.erase(it);
 }
 
-// planar device matrix rows x cols, row-major, named buffer
-template <typename T>
-CMat<T> dmat(rgf_context* c, const std::string& name, int64_t rows, int64_t cols) {
-    T* p = dbuf<T>(c, name, (size_t)2 * std::max<int64_t>(rows, 1) * std::max<int64_t>(cols, 1));
-    return CMat<T>(p, p + (size_t)std::max<int64_t>(rows, 1) * std::max<int64_t>(cols, 1), std::max<int64_t>(cols, 1));
-}
 
 // host complex row-major -> planar device
 void upload(rgf_context* c, const std::vector<cd>& h, int64_t rows, int64_t cols, CMat<double> d) {
@@ -181,32 +142,6 @@ double check_status(rgf_context* c, bool growth = true, bool fp64 = true) {
 
 void reset_status(rgf_context* c) { RGF_HIP(hipMemsetAsync(c->d_status, 0, 2 * sizeof(int), c->stream)); }
 
-// ---------------------------------------------------------------- problem on the device
-// Everything in sweep order: "L" is the lead the slice loop starts from.
-struct Work {
-    const Channels* ch = nullptr;
-    bool mirrored = false;
-    int64_t N = 0, L = 0;
-    cd k2, epsL, epsR;
-    const Side* sL = nullptr;
-    const Side* sR = nullptr;
-    std::vector<int32_t> inL, inR, outL, outR;  // 0-based, sweep-side
-    // device (FP64)
-    double* eps_re = nullptr;
-    double* eps_im = nullptr;  // N x L planar, slice n at n*N
-    DevCSR A0;
-    CMat<double> G0L, G0R, BL, BR, CL, CR;
-    // host results (FP64, row-major), projected Green's functions (no -2i)
-    std::vector<cd> TLL, TLR, TRL, TRR;
-    double flops = 0;
-    double t_setup = 0, t_slices = 0, t_refine = 0;
-    KernelTimer* timer = nullptr;
-    int refine_iters = 0;
-    double refine_residual = 0;
-    double refine_s_err = 0;  // estimated S error at the stop (refine_s_tol)
-    double max_multiplier = 0;  // largest slice-inverse multiplier
-    double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
-};
 
 void upload_eps(rgf_context* c, const rgf_problem* p, Work& w) {
     const int64_t N = w.N, L = w.L;
@@ -243,12 +178,6 @@ void upload_stencil(rgf_context* c, const Channels& ch, Work& w) {
     RGF_HIP(hipStreamSynchronize(c->stream));
 }
 
-// Transverse mode tables f1 (n1 x n1) and f2 (n2 x n2), interleaved FP64,
-// on the device: phi((i1,i2),(a1,a2)) = f1[i1][a1] f2[i2][a2].
-struct ModeTables {
-    const double* f1;
-    const double* f2;
-};
 
 ModeTables upload_modes(rgf_context* c, const Channels& ch) {
     double* df1 = dbuf<double>(c, "f1", 2 * ch.f1.size());
@@ -963,7 +892,7 @@ void download_colmajor(rgf_context* c, CMat<double> d, int64_t rows, int64_t col
         }
 }
 
-}  // namespace
+}  // namespace rgf
 
 // =================================================================== C ABI
 extern "C" {
@@ -1154,8 +1083,18 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
         if (!h) throw Error(RGF_ERR_INVALID_ARGUMENT, "NULL handle");
         if (count < 0 || (count > 0 && (!p || !s || !S_out))) throw Error(RGF_ERR_INVALID_ARGUMENT, "bad batch");
         if (count == 0) return;
-        const int want = opt && opt->batch_concurrency > 0 ? opt->batch_concurrency : 8;
-        const int nw = std::max(1, std::min<int>(want, count));
+        rgf_options o;
+        rgf_default_options(&o);
+        if (opt) o = *opt;
+        std::vector<rgf_status> sts((size_t)count, RGF_OK);
+        std::vector<std::string> msgs((size_t)count);
+        // the lockstep engine takes the small mixed-precision problems
+        std::vector<int32_t> eng, rest;
+        for (int32_t q = 0; q < count; ++q) (batch_engine_takes(&p[q], o) ? eng : rest).push_back(q);
+        if (!eng.empty()) solve_batched_engine(h, eng, p, s, o, S_out, info, sts, msgs);
+        const int32_t nrest = (int32_t)rest.size();
+        const int want = o.batch_concurrency > 0 ? o.batch_concurrency : 8;
+        const int nw = std::max(1, std::min<int>(want, nrest));
         while ((int)h->workers.size() < nw) {
             rgf_handle w = nullptr;
             const rgf_status cs = rgf_create(&w, h->device);
@@ -1164,10 +1103,9 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
         }
         // largest problems first (N^2 L): balances the workers and lets each
         // worker's workspace reach its final size on its first problem
-        std::vector<int32_t> order((size_t)count);
+        std::vector<int32_t> order = rest;
         std::vector<double> cost((size_t)count);
-        for (int32_t q = 0; q < count; ++q) {
-            order[(size_t)q] = q;
+        for (int32_t q : rest) {
             const int nd = p[q].ndim == 3 ? 3 : 2;  // invalid ndim is reported by the solve itself
             double n = 1;
             for (int d = 0; d < nd - 1; ++d) n *= (double)std::max<int64_t>(p[q].dims[d], 1);
@@ -1175,21 +1113,20 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
         }
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
         std::atomic<int32_t> next{0};
-        std::vector<rgf_status> sts((size_t)count, RGF_OK);
-        std::vector<std::string> msgs((size_t)count);
         auto run = [&](int wi) {
             rgf_context* c = h->workers[(size_t)wi];
             if (hipSetDevice(c->device) != hipSuccess) {
-                for (int32_t k = next++; k < count; k = next++) sts[(size_t)order[(size_t)k]] = RGF_ERR_DEVICE;
+                for (int32_t k = next++; k < nrest; k = next++) sts[(size_t)order[(size_t)k]] = RGF_ERR_DEVICE;
                 return;
             }
-            for (int32_t k = next++; k < count; k = next++) {
+            for (int32_t k = next++; k < nrest; k = next++) {
                 const int32_t q = order[(size_t)k];
                 sts[(size_t)q] = guard(c, [&] { solve_one(c, &p[q], &s[q], opt, S_out[q], info ? &info[q] : nullptr); });
                 if (sts[(size_t)q] != RGF_OK) msgs[(size_t)q] = c->err;
             }
         };
-        if (nw == 1) {
+        if (nrest == 0) {
+        } else if (nw == 1) {
             run(0);
         } else {
             std::vector<std::thread> th;

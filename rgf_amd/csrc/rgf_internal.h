@@ -176,4 +176,64 @@ void grad_contract(hipStream_t s, int64_t N, int64_t m, CMat<double> psi, CMat<d
 // max |x| over a planar FP64 array (result to host, synchronizes the stream).
 double max_abs(hipStream_t s, int64_t count, const double* re, const double* im, double* dev_scratch);
 
+// ---- batched engine (many independent mixed-precision solves in lockstep) ----
+// One problem of the batch, device-resident.  The transverse size is padded
+// to NP (a multiple of 64) with the identity: the padded slice matrices are
+// diag(M, I), their inverses diag(g, I), and the padded rows of every
+// right-hand side, correction and solution stay zero.  Complex arrays are
+// planar: the re plane of the whole array, then the im plane.
+struct BatchProb {
+    int32_t N, NP, nb, L;           // transverse size, padded size, 64-wide blocks, slices
+    int32_t m, mL, mR, npo, nqo;    // right-hand sides (mL left + mR right inputs, padded to 4), outputs L / R
+    int32_t pad0;
+    const int32_t* rowptr;          // slice stencil A0 (CSR, N rows, FP64)
+    const int32_t* col;
+    const double* vre;
+    const double* vim;
+    const double* eps_re;           // N x L, slice n at n N; eps_im may be null
+    const double* eps_im;
+    double k2r, k2i, eLr, eLi, eRr, eRi;
+    float* F;                       // factor store: L + 2 slices of 2 x NP x NP
+    float* ws;                      // D (2 x 64 x 64), R (2 x 64 x NP), C (2 x NP x 64)
+    int32_t* status;                // [0] pivot flags, [1] max multiplier^2 (float bits)
+    double* G0L;                    // 2 x NP x NP, zero outside N x N
+    double* G0R;
+    double* B;                      // 2 x NP x m: B_L in columns [0, mL), B_R in [mL, mL + mR)
+    double* CL;                     // 2 x npo x NP
+    double* CR;                     // 2 x nqo x NP
+    double* psi;                    // 2 x (L + 2) x NP x m
+    float* z;                       // 2 x (L + 2) x NP x m: forward z_n, then the correction
+    float* t;                       // 2 x NP x m
+    double* W;                      // 2 x 2 x NP x m: G0L psi_0 (re, im), then G0R psi_{L+1}
+    double* T;                      // 2 x (npo + nqo) x m: C_L psi_0 and C_R psi_{L+1}
+    unsigned long long* rmax;       // max |r| of the last residual (double bits)
+};
+
+// Work maps: entry k of a map is (problem << 20) | local index.
+// Dyson rounds (round r read as *rbase + k): k1 = diagonal-block inverses
+// (work groups [0, np)) and the write-back / assembly chunks (k1map, n1);
+// k2 = panel products (k2map, n2 tiles); k3 = trailing updates (k3map, n3).
+void batch_round(hipStream_t s, const BatchProb* P, int np, const int* rbase, int k, const int32_t* k1map, int n1,
+                 const int32_t* k2map, int n2, const int32_t* k3map, int n3, int use3m);
+// Refinement.  map entries index the sweep-product tiles (NP/64 x m/64 per
+// problem) or the 16384-element chunks of an elementwise pass.
+// residual maximum: first = psi is zero (only the lead slices); chunks over
+// the slices' N x m entries.
+void batch_resmax(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first);
+// lead terms W = G0 psi at the two lead slices (FP64 products)
+void batch_lead_terms(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles);
+// forward step n: t = (float)(r_n / rmax) + z_{n-1}; then z_n = g_n t
+void batch_fwd_rhs(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int n, int first);
+void batch_fwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int n, int use3m);
+// backward step k: z_n += g_n z_{n+1} with n = L - k (each problem's own end)
+void batch_bwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int k, int use3m);
+// psi += rmax z over all slices (first: psi = rmax z)
+void batch_accumulate(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first);
+// T = [C_L psi_0; C_R psi_{L+1}]
+void batch_project(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles);
+// F[0] = (float) G0L with the identity on the padding
+void batch_first_slice(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks);
+void batch_advance(hipStream_t s, int* rbase, int k);
+constexpr int kBatchChunk = 16384;  // elements per work group of the elementwise passes
+
 }  // namespace rgf

@@ -129,7 +129,11 @@ struct GemmArgs {
 __device__ __forceinline__ void wave_amax(unsigned* amax, float m) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(isnan(m) ? INFINITY : m));
+    // only a value above the current maximum needs the atomic (almost every
+    // wavefront skips it: one address takes all the updates of a launch)
+    const unsigned v = __float_as_uint(isnan(m) ? INFINITY : m);
+    if ((threadIdx.x & 63) == 0 && v > __hip_atomic_load(amax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(amax, v);
 }
 
 // A 1D grid covers the tiles of g0 followed by those of g1: two independent
@@ -458,17 +462,13 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm_kernel(GemmArgs<T> g0, Gemm
 //    run column-major in groups of 8 tile rows; MAP 0 keeps the row-major
 //    order with round-robin XCDs (measured equal or faster: the operand
 //    panels of the shapes used here stay in L2 either way).
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR, bool M3>
-__global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, GemmArgs<T> g1, GemmArgs<T> g2,
-                                                              int total) {
-    int bid = (int)blockIdx.x;
-    if constexpr ((MAP & 1) != 0) {
-        const int q = total >> 3, rem = total & 7, x = bid & 7, loc = bid >> 3;
-        bid = x * q + min(x, rem) + loc;
-    }
-    const int which = bid < g0.tiles ? 0 : (bid < g0.tiles + g1.tiles ? 1 : 2);
-    const GemmArgs<T>& g = which == 0 ? g0 : (which == 1 ? g1 : g2);
-    int lin = bid - (which > 0 ? g0.tiles : 0) - (which > 1 ? g1.tiles : 0);
+// One output tile (linear index lin of g's tile grid, split-K chunk
+// included) of the streamlined kernel; the whole work group takes part.
+// AMAX: also track max |C_ij|^2 into g.amax (the column panels of the
+// slice inverse; a separate instantiation keeps the registers of the other
+// products, the trailing update among them, at four waves per SIMD).
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR, bool M3, bool AMAX = false>
+__device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
     int z = 0;
     if (g.nz > 1) {
         const int per = g.tiles / g.nz;
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
     }
 
     // epilogue (the accumulators started from C when beta = 1)
-    unsigned* const amax = g.amax;
+    unsigned* const amax = AMAX ? g.amax : nullptr;
     float mloc = 0.f;
     if constexpr (TR) {
         const bool vc = (ldc % 4) == 0 && (((uintptr_t)Cre | (uintptr_t)Cim) & 15) == 0;
@@ -796,7 +796,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                         *(VT*)(Cre + o + u) = xr;
                         *(VT*)(Cim + o + u) = xi;
                     }
-                    if (amax) {
+                    if (AMAX && amax) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) mloc = fmaxf(mloc, (float)(vr[r] * vr[r] + vi[r] * vi[r]));
                     }
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                         if (c < N && !(c >= g.sclo && c < g.schi) && !(rblk && c >= g.sblo && c < g.sbhi)) {
                             Cre[o + r] = vr[r];
                             Cim[o + r] = vi[r];
-                            if (amax) mloc = fmaxf(mloc, (float)(vr[r] * vr[r] + vi[r] * vi[r]));
+                            if (AMAX && amax) mloc = fmaxf(mloc, (float)(vr[r] * vr[r] + vi[r] * vi[r]));
                         }
                     }
                 }
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     }
                 }
             }
-        if (amax) wave_amax(amax, mloc);
+        if (AMAX && amax) wave_amax(amax, mloc);
         return;
     }
 #pragma unroll
@@ -842,7 +842,7 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                     if constexpr (ACC3) vr -= ii[a][b][r];
                     Cre[(int64_t)gi * ldc + gj] = vr;
                     Cim[(int64_t)gi * ldc + gj] = ri[a][b][r];
-                    if (amax) mloc = fmaxf(mloc, (float)(vr * vr + ri[a][b][r] * ri[a][b][r]));
+                    if (AMAX && amax) mloc = fmaxf(mloc, (float)(vr * vr + ri[a][b][r] * ri[a][b][r]));
                     if (gi >= g.y_rlo && gi < g.y_rhi && gj >= g.y_clo && gj < g.y_chi) {
                         const int64_t oy = (int64_t)(gi - g.y_rlo) * g.ldy + (gj - g.y_clo);
                         g.Yre[oy] = vr;
@@ -851,7 +851,28 @@ __global__ __launch_bounds__(WM* WN * 64) void cgemm2_kernel(GemmArgs<T> g0, Gem
                 }
             }
         }
-    if (amax) wave_amax(amax, mloc);
+    if (AMAX && amax) wave_amax(amax, mloc);
+}
+
+// FP32 64x64 tiles: four work groups per CU (four waves per SIMD, at most
+// 128 registers); the three-product accumulators need all of them.
+template <typename T, int WM, int WN>
+constexpr int gemm_min_blocks() {
+    return (sizeof(T) == 4 && WM * WN == 4) ? 4 : 1;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR, bool M3, bool AMAX = false>
+__global__ __launch_bounds__(WM* WN * 64, (gemm_min_blocks<T, WM, WN>())) void cgemm2_kernel(
+    GemmArgs<T> g0, GemmArgs<T> g1, GemmArgs<T> g2, int total) {
+    int bid = (int)blockIdx.x;
+    if constexpr ((MAP & 1) != 0) {
+        const int q = total >> 3, rem = total & 7, x = bid & 7, loc = bid >> 3;
+        bid = x * q + min(x, rem) + loc;
+    }
+    const int which = bid < g0.tiles ? 0 : (bid < g0.tiles + g1.tiles ? 1 : 2);
+    const GemmArgs<T>& g = which == 0 ? g0 : (which == 1 ? g1 : g2);
+    const int lin = bid - (which > 0 ? g0.tiles : 0) - (which > 1 ? g1.tiles : 0);
+    cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g, lin);
 }
 
 template <typename T>
@@ -961,13 +982,15 @@ static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T
         if (stop) RGF_HIP(hipEventRecord(stop, s));
         return true;
     }
+    const bool am = g0.amax || (g1_ && g1.amax) || (g2_ && g2.amax);
+    auto kern = am ? cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, true>
+                   : cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, false>;
     if (stop) {
         // the event completes with the kernel (no separate marker packet)
-        hipExtLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3>), dim3((unsigned)nblocks),
-                              dim3(WM * WN * 64), 0, s, nullptr, stop, 0, g0, g1, g2, (int)nblocks);
+        hipExtLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(WM * WN * 64), 0, s, nullptr, stop, 0, g0, g1, g2,
+                              (int)nblocks);
     } else {
-        hipLaunchKernelGGL((cgemm2_kernel<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3>), dim3((unsigned)nblocks),
-                           dim3(WM * WN * 64), 0, s, g0, g1, g2, (int)nblocks);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(WM * WN * 64), 0, s, g0, g1, g2, (int)nblocks);
     }
     return true;
 }
@@ -1326,9 +1349,10 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
+// The whole work group (NB / 16 wavefronts) takes part.
 template <typename T, int NB>
-__global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb,
-                                                                   T* Dre, T* Dim, int64_t ldd, int* status) {
+__device__ __forceinline__ void gj_diag_blk(T* Are, T* Aim, int64_t lda, int p0, int nb, T* Dre, T* Dim,
+                                            int64_t ldd, int* status) {
     using V = typename Mfma<T>::v4;
     constexpr int NW = NB / 16;  // wavefronts; wavefront w owns block column w
     constexpr int NT = NW * 64;
@@ -1784,6 +1808,12 @@ __global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Ai
         }
     }
     if (tid == 0 && bad_s) atomicOr(status, 1);
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(NB / 16 * 64) void gj_diag_blk_kernel(T* Are, T* Aim, int64_t lda, int p0, int nb,
+                                                                   T* Dre, T* Dim, int64_t ldd, int* status) {
+    gj_diag_blk<T, NB>(Are, Aim, lda, p0, nb, Dre, Dim, ldd, status);
 }
 
 // Partial write-back of block [p0, p0+nb): with rows != 0, the row panel
@@ -2542,5 +2572,574 @@ template void convert<float, double>(hipStream_t, int64_t, int64_t, CMat<double>
 template void convert<double, float>(hipStream_t, int64_t, int64_t, CMat<float>, CMat<double>, double);
 template void convert<double, double>(hipStream_t, int64_t, int64_t, CMat<double>, CMat<double>, double);
 template void convert<float, float>(hipStream_t, int64_t, int64_t, CMat<float>, CMat<float>, double);
+
+}  // namespace rgf
+
+// ==================================================================== batched engine
+// Many independent mixed-precision solves in lockstep (BatchProb tables).
+// Work maps hold (problem << 20) | local index.
+namespace rgf {
+
+__device__ __forceinline__ int map_prob(int32_t e) { return e >> 20; }
+__device__ __forceinline__ int map_loc(int32_t e) { return e & 0xfffff; }
+
+// GemmArgs of a plain product inside a kernel (every tile interior: the
+// batched shapes are multiples of 64, rows 16-byte aligned)
+template <typename T>
+__device__ __forceinline__ GemmArgs<T> dev_args(int M, int N, int K, const T* Ar, const T* Ai, int64_t lda, const T* Br,
+                                                const T* Bi, int64_t ldb, T* Cr, T* Ci, int64_t ldc, double alpha,
+                                                int beta, int tiles_n) {
+    GemmArgs<T> g;
+    g.Are = Ar;
+    g.Aim = Ai;
+    g.lda = lda;
+    g.Bre = Br;
+    g.Bim = Bi;
+    g.ldb = ldb;
+    g.Cre = Cr;
+    g.Cim = Ci;
+    g.ldc = ldc;
+    g.Xre = Cr;
+    g.Xim = Ci;
+    g.ldx = ldc;
+    g.Yre = nullptr;
+    g.Yim = nullptr;
+    g.ldy = 0;
+    g.y_rlo = g.y_rhi = g.y_clo = g.y_chi = 0;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.alpha_re = alpha;
+    g.alpha_im = 0.0;
+    g.beta = beta;
+    g.srlo = g.srhi = g.sclo = g.schi = 0;
+    g.sblo = g.sbhi = 0;
+    g.vec_ok = 1;
+    g.nz = 1;
+    g.kc = K;
+    g.zstride = 0;
+    g.tiles_n = tiles_n;
+    g.tiles = 0;
+    g.amax = nullptr;
+    return g;
+}
+
+// Panel workspace of a problem: D (64 x 64), R (64 x NP), C (NP x 64), planar.
+__device__ __forceinline__ float* ws_D(const BatchProb& pb) { return pb.ws; }
+__device__ __forceinline__ float* ws_R(const BatchProb& pb) { return pb.ws + 2 * 64 * 64; }
+__device__ __forceinline__ float* ws_C(const BatchProb& pb) { return pb.ws + 2 * 64 * 64 + 2 * 64 * (size_t)pb.NP; }
+
+// g_{n-1}(i, j) as it stands once the write-back of block [p0w, p0w + 64) is
+// done: entries with exactly one index in the block come from the panel
+// workspace (R for the block's rows, C for its columns) while `pending`.
+__device__ __forceinline__ void prev_entry(const BatchProb& pb, const float* Pr, const float* Pi, bool pending,
+                                           int p0w, int i, int j, float& vr, float& vi) {
+    const int NP = pb.NP;
+    const bool ri = i >= p0w && i < p0w + 64, cj = j >= p0w && j < p0w + 64;
+    if (pending && ri != cj) {
+        if (ri) {
+            const float* R = ws_R(pb);
+            vr = R[(size_t)(i - p0w) * NP + j];
+            vi = R[(size_t)64 * NP + (size_t)(i - p0w) * NP + j];
+        } else {
+            const float* C = ws_C(pb);
+            vr = C[(size_t)i * 64 + (j - p0w)];
+            vi = C[(size_t)NP * 64 + (size_t)i * 64 + (j - p0w)];
+        }
+    } else {
+        vr = Pr[(size_t)i * NP + j];
+        vi = Pi[(size_t)i * NP + j];
+    }
+}
+
+// Entry (i, j) of the padded slice matrix of slice n (1..L+1) given
+// g_{n-1}(i, j): A0 - k^2 diag(eps_n) - g_{n-1}, the right lead's
+// A0 - k^2 eps_R - G0R - g_L at n = L + 1, the identity on the padding;
+// summed in FP64, rounded once.
+__device__ __forceinline__ void slice_entry(const BatchProb& pb, int n, int i, int j, float gr, float gi, float& outr,
+                                            float& outi) {
+    if (i >= pb.N || j >= pb.N) {
+        outr = (i == j) ? 1.f : 0.f;
+        outi = 0.f;
+        return;
+    }
+    double vr = -(double)gr, vi = -(double)gi;
+    const size_t NP = pb.NP;
+    if (n == pb.L + 1) {
+        const size_t o = (size_t)i * NP + j;
+        vr -= pb.G0R[o];
+        vi -= pb.G0R[NP * NP + o];
+    }
+    if (i == j) {
+        double er, ei;
+        if (n <= pb.L) {
+            er = pb.eps_re[(size_t)(n - 1) * pb.N + i];
+            ei = pb.eps_im ? pb.eps_im[(size_t)(n - 1) * pb.N + i] : 0.0;
+        } else {
+            er = pb.eRr;
+            ei = pb.eRi;
+        }
+        vr -= pb.k2r * er - pb.k2i * ei;
+        vi -= pb.k2r * ei + pb.k2i * er;
+    }
+    for (int q = pb.rowptr[i]; q < pb.rowptr[i + 1]; ++q)
+        if (pb.col[q] == j) {
+            vr += pb.vre[q];
+            vi += pb.vim[q];
+        }
+    outr = (float)vr;
+    outi = (float)vi;
+}
+
+// Round r, first launch.  Work groups [0, np): the diagonal-block inverse of
+// problem w's step r (slice n = 1 + r / nb, block q = r % nb; at q = 0 the
+// group first assembles that block of the slice matrix).  The others: chunks
+// of the pending write-back of step r - 1 (rows and columns of its block
+// from R and C) and, at q = 0, the assembly of the rest of slice n (which
+// folds in the write-back of slice n - 1's last block).  Step (L + 1) nb
+// only writes back.
+__global__ __launch_bounds__(256) void batch_k1_kernel(const BatchProb* P, int np, const int* rbase, int k,
+                                                       const int32_t* map1) {
+    const int r = *rbase + k;
+    const int w = (int)blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    if (w < np) {
+        const BatchProb& pb = P[w];
+        const int nb = pb.nb, NP = pb.NP;
+        if (r >= (pb.L + 1) * nb) return;
+        const int n = 1 + r / nb, q = r % nb;
+        const size_t NN = (size_t)NP * NP;
+        float* Fr = pb.F + (size_t)n * 2 * NN;
+        float* Fi = Fr + NN;
+        if (q == 0) {
+            // block (0, 0) of slice n; for nb > 1 it lies outside the pending
+            // write-back's rows and columns, for nb = 1 nothing is pending
+            const float* Pr = Fr - 2 * NN;
+            const float* Pi = Pr + NN;
+            for (int e = tid; e < 64 * 64; e += 256) {
+                const int i = e >> 6, j = e & 63;
+                float gr, gi, vr, vi;
+                prev_entry(pb, Pr, Pi, false, 0, i, j, gr, gi);
+                slice_entry(pb, n, i, j, gr, gi, vr, vi);
+                Fr[(size_t)i * NP + j] = vr;
+                Fi[(size_t)i * NP + j] = vi;
+            }
+            __syncthreads();
+        }
+        gj_diag_blk<float, 64>(Fr, Fi, NP, q * 64, 64, ws_D(pb), ws_D(pb) + 64 * 64, 64, pb.status);
+        return;
+    }
+    const int32_t e0 = map1[w - np];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int c = map_loc(e0);
+    const int nb = pb.nb, NP = pb.NP, T = (pb.L + 1) * nb;
+    if (r > T) return;
+    const bool asmb = r < T && (r % nb) == 0;
+    const bool pending = r >= 1 && nb > 1;
+    const int64_t count = asmb ? (int64_t)NP * NP : (pending ? (int64_t)128 * (NP - 64) : 0);
+    const int64_t lo = (int64_t)c * kBatchChunk;
+    if (lo >= count) return;
+    const int64_t hi = min(lo + (int64_t)kBatchChunk, count);
+    const size_t NN = (size_t)NP * NP;
+    const int ps = r - 1;                            // the pending step
+    const int p0w = pending ? (ps % nb) * 64 : 0;
+    float* Wr = pb.F + (size_t)(pending ? 1 + ps / nb : 0) * 2 * NN;  // its slice
+    float* Wi = Wr + NN;
+    if (asmb) {
+        const int n = 1 + r / nb;
+        float* Fr = pb.F + (size_t)n * 2 * NN;
+        float* Fi = Fr + NN;
+        const float* Pr = Fr - 2 * NN;  // slice n - 1 (= the pending slice when pending)
+        const float* Pi = Pr + NN;
+        for (int64_t e = lo + tid; e < hi; e += 256) {
+            const int i = (int)(e / NP), j = (int)(e % NP);
+            if (i < 64 && j < 64) continue;  // the diagonal group's block
+            float gr, gi, vr, vi;
+            prev_entry(pb, Pr, Pi, pending, p0w, i, j, gr, gi);
+            const bool ri = i >= p0w && i < p0w + 64, cj = j >= p0w && j < p0w + 64;
+            if (pending && ri != cj) {
+                Wr[(size_t)i * NP + j] = gr;
+                Wi[(size_t)i * NP + j] = gi;
+            }
+            slice_entry(pb, n, i, j, gr, gi, vr, vi);
+            Fr[(size_t)i * NP + j] = vr;
+            Fi[(size_t)i * NP + j] = vi;
+        }
+        return;
+    }
+    const float* R = ws_R(pb);
+    const float* C = ws_C(pb);
+    const int64_t half = (int64_t)64 * (NP - 64);
+    for (int64_t e = lo + tid; e < hi; e += 256) {
+        int i, j;
+        float vr, vi;
+        if (e < half) {
+            const int a = (int)(e / (NP - 64)), jj = (int)(e % (NP - 64));
+            i = p0w + a;
+            j = jj < p0w ? jj : jj + 64;
+            vr = R[(size_t)a * NP + j];
+            vi = R[(size_t)64 * NP + (size_t)a * NP + j];
+        } else {
+            const int64_t e2 = e - half;
+            const int ii = (int)(e2 >> 6), b = (int)(e2 & 63);
+            i = ii < p0w ? ii : ii + 64;
+            j = p0w + b;
+            vr = C[(size_t)i * 64 + b];
+            vi = C[(size_t)NP * 64 + (size_t)i * 64 + b];
+        }
+        Wr[(size_t)i * NP + j] = vr;
+        Wi[(size_t)i * NP + j] = vi;
+    }
+}
+
+// Round r, second launch: R = D A[P, :] (columns P skipped) and
+// C = -A[:, P] D (rows P skipped; its largest entry is the step's largest
+// multiplier) of every problem's step r.  Tiles [0, nb) of a problem are R's,
+// [nb, 2 nb) C's.
+template <bool M3>
+__global__ __launch_bounds__(256) void batch_k2_kernel(const BatchProb* P, const int* rbase, int k,
+                                                       const int32_t* map2) {
+    const int r = *rbase + k;
+    const int32_t e0 = map2[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int t = map_loc(e0);
+    const int nb = pb.nb, NP = pb.NP;
+    if (r >= (pb.L + 1) * nb) return;
+    const int n = 1 + r / nb, p0 = (r % nb) * 64;
+    const size_t NN = (size_t)NP * NP;
+    float* Fr = pb.F + (size_t)n * 2 * NN;
+    float* Fi = Fr + NN;
+    float* D = ws_D(pb);
+    float* R = ws_R(pb);
+    float* C = ws_C(pb);
+    if (t < nb) {
+        GemmArgs<float> g = dev_args<float>(64, NP, 64, D, D + 64 * 64, 64, Fr + (size_t)p0 * NP,
+                                            Fi + (size_t)p0 * NP, NP, R, R + (size_t)64 * NP, NP, 1.0, 0, nb);
+        g.sclo = p0;
+        g.schi = p0 + 64;
+        cgemm2_tile<float, 64, 64, 16, 2, 2, false, 0, true, M3, true>(g, t);
+    } else {
+        GemmArgs<float> g = dev_args<float>(NP, 64, 64, Fr + p0, Fi + p0, NP, D, D + 64 * 64, 64, C,
+                                            C + (size_t)NP * 64, 64, -1.0, 0, 1);
+        g.srlo = p0;
+        g.srhi = p0 + 64;
+        g.amax = (unsigned*)(pb.status + 1);
+        cgemm2_tile<float, 64, 64, 16, 2, 2, false, 0, true, M3, true>(g, t - nb);
+    }
+}
+
+// Round r, third launch: the trailing update A[I, J] -= A[I, P] R[P, J] of
+// every problem's step r over the (nb - 1)^2 tiles outside block row and
+// column P (A[:, P] still holds the old column: it is written back in the
+// next round's first launch).
+template <bool M3>
+__global__ __launch_bounds__(256, 4) void batch_k3_kernel(const BatchProb* P, const int* rbase, int k,
+                                                          const int32_t* map3) {
+    const int r = *rbase + k;
+    const int32_t e0 = map3[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int t = map_loc(e0);
+    const int nb = pb.nb, NP = pb.NP;
+    if (r >= (pb.L + 1) * nb) return;
+    const int n = 1 + r / nb, q = r % nb, p0 = q * 64;
+    const int ti0 = t / (nb - 1), tj0 = t % (nb - 1);
+    const int ti = ti0 + (ti0 >= q ? 1 : 0), tj = tj0 + (tj0 >= q ? 1 : 0);
+    const size_t NN = (size_t)NP * NP;
+    float* Fr = pb.F + (size_t)n * 2 * NN;
+    float* Fi = Fr + NN;
+    float* R = ws_R(pb);
+    const GemmArgs<float> g = dev_args<float>(NP, NP, 64, Fr + p0, Fi + p0, NP, R, R + (size_t)64 * NP, NP, Fr, Fi,
+                                              NP, -1.0, 1, nb);
+    cgemm2_tile<float, 64, 64, 16, 2, 2, false, 0, true, M3, false>(g, ti * nb + tj);
+}
+
+__global__ void batch_advance_kernel(int* rbase, int k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *rbase += k;
+}
+
+// F[0] = (float) G0L, the identity on the padding
+__global__ __launch_bounds__(256) void batch_first_slice_kernel(const BatchProb* P, const int32_t* map) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int NP = pb.NP, N = pb.N;
+    const int64_t NN = (int64_t)NP * NP;
+    const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, NN);
+    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
+        const int i = (int)(e / NP), j = (int)(e % NP);
+        float vr = (i == j) ? 1.f : 0.f, vi = 0.f;
+        if (i < N && j < N) {
+            vr = (float)pb.G0L[e];
+            vi = (float)pb.G0L[NN + e];
+        }
+        pb.F[e] = vr;
+        pb.F[NN + e] = vi;
+    }
+}
+
+// ------------------------------------------------------------------ batched refinement
+// Residual r = b - H psi at slice n, transverse site i (< N), column c:
+// b holds B_L at slice 0 (columns < mL) and B_R at slice L + 1 (columns
+// [mL, mL + mR)); H psi = (A0 - k^2 eps_n) psi_n - psi_{n-1} - psi_{n+1},
+// with the lead terms G0L psi_0 and G0R psi_{L+1} (W) at the end slices.
+// zero_psi: psi = 0 (the first correction), r = b.
+__device__ __forceinline__ void batch_residual(const BatchProb& pb, int n, int i, int c, bool zero_psi, double& rr,
+                                               double& ri) {
+    const int m = pb.m, S = pb.L + 2;
+    const size_t SE = (size_t)pb.NP * m;
+    const size_t o = (size_t)i * m + c;
+    rr = 0.0;
+    ri = 0.0;
+    if ((n == 0 && c < pb.mL) || (n == S - 1 && c >= pb.mL && c < pb.mL + pb.mR)) {
+        rr = pb.B[o];
+        ri = pb.B[SE + o];
+    }
+    if (zero_psi) return;
+    const double* pr = pb.psi;
+    const double* pim = pb.psi + (size_t)S * SE;
+    double er, ei;
+    if (n == 0) {
+        er = pb.eLr;
+        ei = pb.eLi;
+    } else if (n == S - 1) {
+        er = pb.eRr;
+        ei = pb.eRi;
+    } else {
+        er = pb.eps_re[(size_t)(n - 1) * pb.N + i];
+        ei = pb.eps_im ? pb.eps_im[(size_t)(n - 1) * pb.N + i] : 0.0;
+    }
+    const double ker = pb.k2r * er - pb.k2i * ei, kei = pb.k2r * ei + pb.k2i * er;
+    const size_t e = (size_t)n * SE + o;
+    const double psr = pr[e], psi = pim[e];
+    double ar = ker * psr - kei * psi, ai = ker * psi + kei * psr;
+    for (int q = pb.rowptr[i]; q < pb.rowptr[i + 1]; ++q) {
+        const size_t oq = (size_t)n * SE + (size_t)pb.col[q] * m + c;
+        const double vr = pb.vre[q], vi = pb.vim[q];
+        ar -= vr * pr[oq] - vi * pim[oq];
+        ai -= vr * pim[oq] + vi * pr[oq];
+    }
+    if (n > 0) {
+        ar += pr[e - SE];
+        ai += pim[e - SE];
+    }
+    if (n < S - 1) {
+        ar += pr[e + SE];
+        ai += pim[e + SE];
+    }
+    if (n == 0) {
+        ar += pb.W[o];
+        ai += pb.W[SE + o];
+    }
+    if (n == S - 1) {
+        ar += pb.W[2 * SE + o];
+        ai += pb.W[3 * SE + o];
+    }
+    rr += ar;
+    ri += ai;
+}
+
+// max |r| per problem over its slices (first: psi = 0, the lead slices only)
+__global__ __launch_bounds__(256) void batch_resmax_kernel(const BatchProb* P, const int32_t* map, int first) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int64_t per = (int64_t)pb.N * pb.m;
+    const int S = pb.L + 2;
+    const int64_t count = first ? 2 * per : (int64_t)S * per;
+    const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, count);
+    double mx = 0.0;
+    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
+        const int64_t sl = e / per, rem = e - sl * per;
+        const int n = first ? (sl ? S - 1 : 0) : (int)sl;
+        const int i = (int)(rem / pb.m), c = (int)(rem % pb.m);
+        double rr, ri;
+        batch_residual(pb, n, i, c, first != 0, rr, ri);
+        const double v = sqrt(rr * rr + ri * ri);
+        if (!(v <= mx)) mx = v;  // NaN propagates
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(mx, off);
+        if (!(o <= mx)) mx = o;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned long long bits =
+            isnan(mx) ? 0x7ff8000000000000ULL : (unsigned long long)__double_as_longlong(mx);
+        atomicMax(pb.rmax, bits);
+    }
+}
+
+// forward step n: t = (float)(r_n / rmax) + z_{n-1} (t's padding rows stay zero)
+__global__ __launch_bounds__(256) void batch_fwd_rhs_kernel(const BatchProb* P, const int32_t* map, int n, int first) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int S = pb.L + 2;
+    if (n >= S) return;
+    const int64_t per = (int64_t)pb.N * pb.m;
+    const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, per);
+    const double sc = 1.0 / __longlong_as_double((long long)*pb.rmax);
+    const size_t SE = (size_t)pb.NP * pb.m;
+    float* tr = pb.t;
+    float* ti = pb.t + SE;
+    const float* zr = pb.z + (size_t)(n - 1) * SE;
+    const float* zi = pb.z + (size_t)S * SE + (size_t)(n - 1) * SE;
+    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
+        const int i = (int)(e / pb.m), c = (int)(e % pb.m);
+        double rr, ri;
+        batch_residual(pb, n, i, c, first != 0, rr, ri);
+        float vr = (float)(rr * sc), vi = (float)(ri * sc);
+        if (n > 0) {
+            vr += zr[e];
+            vi += zi[e];
+        }
+        tr[e] = vr;
+        ti[e] = vi;
+    }
+}
+
+// z_n = g_n t (forward) or z_n += g_n z_{n+1} with n = L - k (backward):
+// NP x m products, K = NP, tiles NP/64 x ceil(m/64)
+template <bool M3, bool BWD>
+__global__ __launch_bounds__(256, 4) void batch_sweep_gemm_kernel(const BatchProb* P, const int32_t* map, int step) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int S = pb.L + 2, NP = pb.NP, m = pb.m;
+    const int n = BWD ? S - 2 - step : step;
+    if (n < 0 || n >= S) return;
+    const size_t NN = (size_t)NP * NP, SE = (size_t)NP * m;
+    const float* Fr = pb.F + (size_t)n * 2 * NN;
+    const float* Fi = Fr + NN;
+    float* zr = pb.z + (size_t)n * SE;
+    float* zi = pb.z + (size_t)S * SE + (size_t)n * SE;
+    const float* br = BWD ? zr + SE : pb.t;
+    const float* bi = BWD ? zi + SE : pb.t + SE;
+    const GemmArgs<float> g =
+        dev_args<float>(NP, m, NP, Fr, Fi, NP, br, bi, m, zr, zi, m, 1.0, BWD ? 1 : 0, (m + 63) / 64);
+    cgemm2_tile<float, 64, 64, 16, 2, 2, false, 0, true, M3, false>(g, map_loc(e0));
+}
+
+// W0 = G0L psi_0, W1 = G0R psi_{L+1} (FP64); tiles [0, h) W0's, [h, 2h) W1's
+__global__ __launch_bounds__(256) void batch_lead_kernel(const BatchProb* P, const int32_t* map) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int S = pb.L + 2, NP = pb.NP, m = pb.m;
+    const int tn = (m + 63) / 64, h = (NP / 64) * tn;
+    int t = map_loc(e0);
+    const bool right = t >= h;
+    if (right) t -= h;
+    const size_t NN = (size_t)NP * NP, SE = (size_t)NP * m;
+    const double* G = right ? pb.G0R : pb.G0L;
+    const size_t so = right ? (size_t)(S - 1) * SE : 0;
+    double* W = pb.W + (right ? 2 * SE : 0);
+    const GemmArgs<double> g = dev_args<double>(NP, m, NP, G, G + NN, NP, pb.psi + so, pb.psi + (size_t)S * SE + so, m,
+                                                W, W + SE, m, 1.0, 0, tn);
+    cgemm2_tile<double, 64, 64, 16, 2, 2, false, 0, true, false, false>(g, t);
+}
+
+// T = [C_L psi_0; C_R psi_{L+1}] (FP64); tiles [0, hL) the left block's
+__global__ __launch_bounds__(256) void batch_project_kernel(const BatchProb* P, const int32_t* map) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int S = pb.L + 2, NP = pb.NP, m = pb.m;
+    const int tn = (m + 63) / 64, hL = ((pb.npo + 63) / 64) * tn;
+    int t = map_loc(e0);
+    const bool right = t >= hL;
+    if (right) t -= hL;
+    const int rows = right ? pb.nqo : pb.npo;
+    if (rows == 0) return;
+    const size_t SE = (size_t)NP * m, TE = (size_t)(pb.npo + pb.nqo) * m;
+    const double* Cm = right ? pb.CR : pb.CL;
+    const size_t so = right ? (size_t)(S - 1) * SE : 0;
+    double* T = pb.T + (right ? (size_t)pb.npo * m : 0);
+    const GemmArgs<double> g = dev_args<double>(rows, m, NP, Cm, Cm + (size_t)rows * NP, NP, pb.psi + so,
+                                                pb.psi + (size_t)S * SE + so, m, T, T + TE, m, 1.0, 0, tn);
+    cgemm2_tile<double, 64, 64, 16, 2, 2, false, 0, true, false, false>(g, t);
+}
+
+// psi += rmax z over every slice (first: psi = rmax z, psi was never written)
+__global__ __launch_bounds__(256) void batch_accumulate_kernel(const BatchProb* P, const int32_t* map, int first) {
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int64_t tot = (int64_t)(pb.L + 2) * pb.NP * pb.m;
+    const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, tot);
+    const double sc = __longlong_as_double((long long)*pb.rmax);
+    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
+        const double a = (double)pb.z[e] * sc, b = (double)pb.z[tot + e] * sc;
+        pb.psi[e] = first ? a : pb.psi[e] + a;
+        pb.psi[tot + e] = first ? b : pb.psi[tot + e] + b;
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+void batch_round(hipStream_t s, const BatchProb* P, int np, const int* rbase, int k, const int32_t* k1map, int n1,
+                 const int32_t* k2map, int n2, const int32_t* k3map, int n3, int use3m) {
+    hipLaunchKernelGGL(batch_k1_kernel, dim3((unsigned)(np + n1)), dim3(256), 0, s, P, np, rbase, k, k1map);
+    if (n2 > 0) {
+        if (use3m)
+            hipLaunchKernelGGL((batch_k2_kernel<true>), dim3((unsigned)n2), dim3(256), 0, s, P, rbase, k, k2map);
+        else
+            hipLaunchKernelGGL((batch_k2_kernel<false>), dim3((unsigned)n2), dim3(256), 0, s, P, rbase, k, k2map);
+    }
+    if (n3 > 0) {
+        if (use3m)
+            hipLaunchKernelGGL((batch_k3_kernel<true>), dim3((unsigned)n3), dim3(256), 0, s, P, rbase, k, k3map);
+        else
+            hipLaunchKernelGGL((batch_k3_kernel<false>), dim3((unsigned)n3), dim3(256), 0, s, P, rbase, k, k3map);
+    }
+    RGF_HIP(hipGetLastError());
+}
+
+void batch_advance(hipStream_t s, int* rbase, int k) {
+    hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, s, rbase, k);
+}
+
+void batch_first_slice(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(batch_first_slice_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, P, map);
+    RGF_HIP(hipGetLastError());
+}
+
+void batch_resmax(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(batch_resmax_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, P, map, first);
+    RGF_HIP(hipGetLastError());
+}
+
+void batch_lead_terms(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(batch_lead_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, P, map);
+    RGF_HIP(hipGetLastError());
+}
+
+void batch_fwd_rhs(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int n, int first) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(batch_fwd_rhs_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, P, map, n, first);
+}
+
+void batch_fwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int n, int use3m) {
+    if (ntiles <= 0) return;
+    if (use3m)
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<true, false>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, n);
+    else
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<false, false>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map,
+                           n);
+}
+
+void batch_bwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int k, int use3m) {
+    if (ntiles <= 0) return;
+    if (use3m)
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<true, true>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, k);
+    else
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<false, true>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, k);
+}
+
+void batch_accumulate(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(batch_accumulate_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, P, map, first);
+    RGF_HIP(hipGetLastError());
+}
+
+void batch_project(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(batch_project_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, P, map);
+    RGF_HIP(hipGetLastError());
+}
 
 }  // namespace rgf

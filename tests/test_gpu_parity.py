@@ -432,8 +432,11 @@ def test_batched_ragged_vs_single_and_oracle():
         items.append((syst, *sel))
     eps3 = 1 + rng.random((5, 6, 7))
     items.append((dict(epsilon=eps3, epsilon_L=1.0, epsilon_R=1.2, k0dx=1.1, xyBC=[0.3, 1.7]), ["L", "R"], ["L"]))
+    # fp32_refined with batch_engine 2: every problem on the worker contexts
+    # (the lockstep engine has tests of its own)
     for precision, tol in (("fp64", 1e-10), ("fp32_refined", 1e-9)):
-        Sb = rgf_amd.cal_smatrix_RGF_batched(items, precision=precision, options={"batch_concurrency": 4})
+        Sb = rgf_amd.cal_smatrix_RGF_batched(items, precision=precision,
+                                             options={"batch_concurrency": 4, "batch_engine": 2})
         for (syst, out, inn), S in zip(items, Sb):
             three_d = np.ndim(syst["epsilon"]) == 3
             single = rgf_amd.cal_smatrix_RGF_3D if three_d else rgf_amd.cal_smatrix_RGF
