@@ -92,6 +92,31 @@ def cpu_baseline(syst, n_slices: int = 2):
                       f"at N={N} after 1 warm-up update; {dt:.1f} s"}
 
 
+def cpu_baseline_2d(syst, n_slices: int = 8):
+    """Oracle (numpy/LAPACK FP64) timing of the reference's general slice update
+    (cal_smatrix_RGF.m:263-269: inv + 3 zgemm) for one 2D problem."""
+    import oracle.rgf_oracle as o
+    eps = syst["epsilon"]
+    N = eps.shape[0]
+    A0 = o.slice_operator_2d(N, syst["yBC"])
+    k2 = complex(syst["k0dx"]) ** 2
+    rng = np.random.default_rng(0)
+    G = [(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) * (0.1 / math.sqrt(N)) for _ in range(4)]
+
+    def step(n):
+        A_nn = A0 - np.diag(k2 * eps[:, n])
+        G[0] = np.linalg.inv(A_nn - G[0])
+        G[1] = G[0] @ G[1]
+        G[2] = G[2] + G[3] @ G[1]
+        G[3] = G[3] @ G[0]
+
+    step(0)
+    t0 = time.perf_counter()
+    for n in range(1, 1 + n_slices):
+        step(n)
+    return n_slices / (time.perf_counter() - t0)
+
+
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_slice_inverse_n4096.md")
 
 
@@ -107,7 +132,7 @@ def counters(prec: str):
         return None
 
 
-def roofline(info: dict, prec: str, use3m: bool = False):
+def roofline(info: dict, prec: str, use3m: bool = False, kernel: str | None = None, pmc_key: str | None = "default"):
     """Roofline record of the trailing update.  achieved and frac count the
     flops of the formulation the kernel executes: 8 per complex multiply-add
     with four real products, 6 with three (8 (N-2b)^2 b or 6 (N-2b)^2 b per
@@ -120,7 +145,7 @@ def roofline(info: dict, prec: str, use3m: bool = False):
     executed = info["kernel_flops"]           # as the library counts them: 6 or 8 per multiply-add
     ach = executed / t / 1e12
     effective = (executed * (8.0 / 6.0) if use3m else executed) / t / 1e12
-    pmc = counters(prec + ("_3m" if use3m else ""))
+    pmc = counters(prec + ("_3m" if use3m else "")) if pmc_key == "default" else (counters(pmc_key) if pmc_key else None)
     return {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
             "frac": round(ach / PEAK_TFLOPS[prec], 4),
             "flops_counted": "executed: 6 per complex multiply-add (three real products)" if use3m
@@ -131,7 +156,7 @@ def roofline(info: dict, prec: str, use3m: bool = False):
             "algorithmic_bytes": pmc["algorithmic_bytes"] if pmc else None,
             "mfma_util_pct": pmc["mfma_util_pct"] if pmc else None,
             "counters_from": os.path.relpath(PMC_SUMMARY, ROOT) if pmc else None,
-            "kernel": f"cgemm trailing update of the slice inverse ({prec}{', 3 real products' if use3m else ''})",
+            "kernel": kernel or f"cgemm trailing update of the slice inverse ({prec}{', 3 real products' if use3m else ''})",
             "launches_timed": int(info["kernel_launches"]),
             "avg_us": round(1e6 * t / info["kernel_launches"], 2),
             "flops_per_launch": executed / info["kernel_launches"]}
@@ -162,12 +187,101 @@ def main():
     ap.add_argument("--host-input", action="store_true",
                     help="hand epsilon over as a host array (adds its column-major copy and upload to every step); "
                          "by default it is resident in HBM before the timed region")
+    ap.add_argument("--workload", choices=["spheres3d", "sweep"], default="spheres3d",
+                    help="spheres3d: the 3D N=4096 headline workload (default); sweep: 256 2D solves of widths "
+                         "128..512, 500 slices, complex frequencies, in the lockstep batched engine")
+    ap.add_argument("--compare-workers", action="store_true",
+                    help="sweep: also time the same batch on host-thread worker contexts (batch_engine 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the slab would shard along the slice axis or across independent solves).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sweep":
+        bench_sweep(args)
+    else:
+        bench_spheres(args)
 
+
+def bench_sweep(args):
+    """256 independent 2D solves in one call of the batched engine; value =
+    256 * 500 * steps / elapsed (epsilon resident in HBM as CUDA tensors)."""
+    import torch
+    import rgf_amd
+    from rgf_amd.workloads import batched_sweep
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    host_items = batched_sweep()
+    items = [(dict(s, epsilon=torch.as_tensor(s["epsilon"], device="cuda:0")), out, inn) for s, out, inn in host_items]
+    L = host_items[0][0]["epsilon"].shape[1]
+    count = len(items)
+    opts = {"complex_3m": args.complex_3m}
+
+    def run(profile=0, **kw):
+        return rgf_amd.cal_smatrix_RGF_batched(items, precision="fp32_refined", device=0, return_info=True,
+                                               options=dict(opts, profile=profile, **kw))
+
+    print(f"# workload: batched sweep, {count} solves x {L} slices; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = None
+    for k in range(args.steps):
+        Sb, infos = run(profile=1)
+        i0 = infos[0]
+        print(f"# step {k}: slices {i0['seconds_slices']:.3f} s, refine {i0['seconds_refine']:.3f} s, "
+              f"sweeps {max(i['refine_iters'] for i in infos)}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = count * L * args.steps / dt
+    widths = sorted({s["epsilon"].shape[0] for s, _, _ in host_items})
+    rec = {
+        "metric": "slice-updates/s, batched 2D sweep (256 solves, N 128-512, 500 slices): FP32+refine, one call",
+        "value": round(value, 1), "unit": "slice-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000 * dt / args.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (FP32 MFMA) + c128 refinement",
+        "data": "synthetic (seeded eps ~ U[1,3], lambda ~ U[8,20] px, Im(k0dx)/Re(k0dx) ~ U[0,1e-3])",
+        "config": {"workload": f"{count} 2D solves in one rgf_solve_batched call, widths {widths}, {L} slices, "
+                               "periodic, eps_L = eps_R = 1, full S, lockstep engine with hipGraph rounds",
+                   "solves": count, "slices": L, "parallelism": "single GPU"},
+        "roofline": roofline(infos[0], "fp32", bool(args.complex_3m & 2),
+                             kernel="batched refinement sweep product z_n = g_n t (FP32"
+                                    f"{', 3 real products' if args.complex_3m & 2 else ''})",
+                             pmc_key=None),
+        "epsilon_input": "resident in HBM (CUDA tensors)",
+        "seconds_slices": round(infos[0]["seconds_slices"], 3),
+        "seconds_refine": round(infos[0]["seconds_refine"], 3),
+        "refine_tol": 1e-13, "max_refine_iters": max(i["refine_iters"] for i in infos),
+        "max_refine_residual": max(i["refine_residual"] for i in infos),
+    }
+    if args.compare_workers:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        Sw, _ = run(batch_engine=2)
+        tw = time.perf_counter() - t1
+        rec["worker_contexts"] = {"value": round(count * L / tw, 1), "seconds": round(tw, 3),
+                                  "max_abs_diff": float(max(np.abs(a - b).max() for a, b in zip(Sb, Sw)))}
+    if not args.skip_cpu:
+        big = next(s for s, _, _ in host_items if s["epsilon"].shape[0] == 512)
+        n = max(args.cpu_slices, 8)
+        v = cpu_baseline_2d(big, n)
+        # every width at its own rate, weighted as in the batch: seconds per slice-update summed
+        per = {}
+        for w in widths:
+            sw = next(s for s, _, _ in host_items if s["epsilon"].shape[0] == w)
+            per[w] = v if w == 512 else cpu_baseline_2d(sw, n)
+        tot = sum(1.0 / per[s["epsilon"].shape[0]] for s, _, _ in host_items)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        rec["cpu_baseline"] = {"value": round(count / tot, 2), "unit": "slice-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{n} general-scheme slice updates (inv + 3 zgemm, complex FP64, "
+                                         f"numpy/LAPACK) at each width {widths}, combined over the batch's "
+                                         "mix of widths"}
+    print(json.dumps(rec), flush=True)
+
+
+def bench_spheres(args):
     import torch
     import rgf_amd
     from rgf_amd.workloads import spheres_3d

@@ -127,9 +127,12 @@ typedef struct {
     double refine_residual;   /* final max|b - H psi| / max|b|                          */
     double seconds_total, seconds_setup, seconds_slices, seconds_refine;
     double flops_slices;      /* algorithmic flops executed in the slice loop          */
-    /* with opt.profile: the dominant kernel (rank-b trailing update of the
-       slice inverse), over the launches timed (one in four): summed
-       event-timed duration, number of timed launches, their executed flops */
+    /* with opt.profile: the dominant kernel, over the launches timed (one in
+       four): summed event-timed duration, number of timed launches, their
+       executed flops.  rgf_solve: the rank-b trailing update of the slice
+       inverse; the lockstep batched engine: the refinement sweep product
+       z_n = g_n t over the whole batch (the same figures in every info of
+       the batch) */
     double kernel_seconds;
     double kernel_flops;
     int64_t kernel_launches;

@@ -209,6 +209,11 @@ def _problem_3d(syst) -> _Prob:
                  _scalar(syst["epsilon_R"], "syst.epsilon_R"), _scalar(syst["k0dx"], "syst.k0dx"), bcs, kas)
 
 
+def _count(sides, nprop) -> int:
+    """Channels a parsed selection asks for (rgf_query's n_out / n_in)."""
+    return sum(nprop[k] if isinstance(v, int) else len(v) for k, v in enumerate(sides))
+
+
 def _query(prob: _Prob) -> tuple[int, int]:
     lib = L.load_library()
     nL, nR = C.c_int32(), C.c_int32()
@@ -371,12 +376,10 @@ def cal_smatrix_RGF_batched(items, *, precision: str = "fp64", device: int = -1,
     for syst, out, in_ in items:
         prob = _problem_3d(syst) if np.ndim(syst["epsilon"]) == 3 else _problem_2d(syst)
         nprop = _query(prob)
-        sel = _Sel(_parse_selection(out, "out", nprop), _parse_selection(in_, "in", nprop))
-        nout, nin = C.c_int32(), C.c_int32()
-        L.check(lib.rgf_query(C.byref(prob.p), C.byref(sel.s), None, None, C.byref(nout), C.byref(nin)))
+        so, si = _parse_selection(out, "out", nprop), _parse_selection(in_, "in", nprop)
         probs.append(prob)
-        sels.append(sel)
-        Ss.append(np.zeros((nout.value, nin.value), dtype=np.complex128, order="F"))
+        sels.append(_Sel(so, si))
+        Ss.append(np.zeros((_count(so, nprop), _count(si, nprop)), dtype=np.complex128, order="F"))
     n = len(items)
     if n == 0:
         return ([], []) if return_info else []

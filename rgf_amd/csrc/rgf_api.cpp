@@ -999,7 +999,7 @@ rgf_status rgf_trim(rgf_handle h) {
 rgf_status rgf_query(const rgf_problem* p, const rgf_selection* s, int32_t* nL, int32_t* nR, int32_t* nout,
                      int32_t* nin) {
     return guard(nullptr, [&] {
-        Channels ch = setup_channels(p);
+        Channels ch = setup_channels(p, false);
         if (nL) *nL = ch.L.n_prop;
         if (nR) *nR = ch.R.n_prop;
         if (s) {
@@ -1013,7 +1013,7 @@ rgf_status rgf_query(const rgf_problem* p, const rgf_selection* s, int32_t* nL, 
 rgf_status rgf_setup_channels(const rgf_problem* p, rgf_channels* out) {
     return guard(nullptr, [&] {
         if (!out) throw Error(RGF_ERR_INVALID_ARGUMENT, "channels output is NULL");
-        Channels ch = setup_channels(p);
+        Channels ch = setup_channels(p, false);
         const int64_t N = ch.N;
         for (int64_t a = 0; a < N; ++a) {
             if (out->kt1) out->kt1[a] = ch.kt1[a];

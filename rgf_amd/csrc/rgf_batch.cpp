@@ -28,6 +28,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <thread>
 
 #include "rgf_context.h"
@@ -122,6 +123,9 @@ void setup_one(rgf_context* c, HostProb& hp, char* arena, const rgf_problem* p) 
     char* base = arena + hp.base;
     const int64_t N = hp.N, NP = hp.NP, L = hp.L, m = hp.m;
     RGF_HIP(hipMemsetAsync(base, 0, hp.zero_bytes, s));
+    // the first correction's forward products leave the right inputs' columns
+    // of z untouched before the last slice: they must read as zero
+    RGF_HIP(hipMemsetAsync(base + hp.o_z, 0, sizeof(float) * 2 * hp.S() * NP * m, s));
     double* eps = (double*)(base + hp.o_eps);
     if (L > 0) {
         const hipMemcpyKind kind = p->eps_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
@@ -136,6 +140,7 @@ void setup_one(rgf_context* c, HostProb& hp, char* arena, const rgf_problem* p) 
     RGF_HIP(hipMemcpyAsync(cs + co[1], h.col.data(), 4 * h.col.size(), hipMemcpyHostToDevice, s));
     RGF_HIP(hipMemcpyAsync(cs + co[2], h.vre.data(), 8 * h.vre.size(), hipMemcpyHostToDevice, s));
     RGF_HIP(hipMemcpyAsync(cs + co[3], h.vim.data(), 8 * h.vim.size(), hipMemcpyHostToDevice, s));
+    build_modes(hp.ch);
     const ModeTables f = upload_modes(c, hp.ch);
     const size_t NN = (size_t)NP * NP;
     double* G0L = (double*)(base + hp.o_G0L);
@@ -285,7 +290,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
     // problems without work keep their diagonal groups (they exit at once):
     // the k1 grid indexes the table directly
     auto nn = [&](size_t k) { return ps[k]->NP * ps[k]->NP; };
-    const int n1 = mp.append(live, [&](size_t k) { return ceil_div(nn(k), CH); });
+    const int n1 = mp.append(live, [&](size_t k) { return ps[k]->NP / kBatchRows; });
     const int n2 = mp.append(live, [&](size_t k) { return ps[k]->nb > 1 ? 2 * ps[k]->nb : (int64_t)0; });
     const int n3 = mp.append(live, [&](size_t k) { return (ps[k]->nb - 1) * (ps[k]->nb - 1); });
     const int nfs = mp.append(live, [&](size_t k) { return ceil_div(nn(k), CH); });
@@ -303,17 +308,29 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
     for (size_t k : live) rounds = std::max(rounds, ps[k]->T() + 1);
     const int graph_mode = o.batch_engine;  // 0: hipGraph replays, 1: plain launches
     const int K = o.batch_graph_rounds > 0 ? o.batch_graph_rounds : 16;
+    // fork / join of the row pass on the second stream
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } fork, join;
+    RGF_HIP(hipEventCreateWithFlags(&fork.e, hipEventDisableTiming));
+    RGF_HIP(hipEventCreateWithFlags(&join.e, hipEventDisableTiming));
+    hipStream_t s2 = h->ov.s2;
     RGF_HIP(hipStreamSynchronize(s));
     const double t_setup = now() - t0;
     const double t1 = now();
     if (graph_mode == 1) {
         for (int64_t r = 0; r < rounds; ++r)
-            batch_round(s, dtab, (int)np, rbase, (int)r, k1map, n1, k2map, n2, k3map, n3, use3m);
+            batch_round(s, s2, fork.e, join.e, dtab, (int)np, rbase, (int)r, k1map, n1, k2map, n2, k3map, n3,
+                        use3m);
     } else {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         RGF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < K; ++k) batch_round(s, dtab, (int)np, rbase, k, k1map, n1, k2map, n2, k3map, n3, use3m);
+        for (int k = 0; k < K; ++k)
+            batch_round(s, s2, fork.e, join.e, dtab, (int)np, rbase, k, k1map, n1, k2map, n2, k3map, n3, use3m);
         batch_advance(s, rbase, K);
         RGF_HIP(hipStreamEndCapture(s, &graph));
         const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -348,6 +365,11 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
 
     // refinement
     const double t2 = now();
+    // with opt.profile: one forward sweep product in four timed with HIP
+    // events on the stream (the engine's dominant kernel)
+    std::unique_ptr<KernelTimer> timer;
+    if (o.profile) timer.reset(new KernelTimer(&h->timer_events));
+    const double fl3 = sweep3m ? 6.0 : 8.0;
     const int maxit = o.max_refine_iter > 0 ? o.max_refine_iter : 6;
     const double tol = o.refine_tol > 0 ? o.refine_tol : 1e-13;
     std::vector<unsigned long long> rm(np);
@@ -362,7 +384,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
             return 2 * (ps[k]->NP / 64) * ceil_div(ps[k]->m, 64);
         });
         const int nres = im.append(act, [&](size_t k) {
-            return ceil_div((first ? 2 : ps[k]->S()) * ps[k]->N * ps[k]->m, CH);
+            return ceil_div((first ? 2 : ps[k]->S()) * ps[k]->N, kBatchResRows);
         });
         int32_t* imaps = dbuf<int32_t>(h, "batch_imaps", im.v.size() + 64);
         RGF_HIP(hipMemcpyAsync(imaps, im.v.data(), 4 * im.v.size(), hipMemcpyHostToDevice, s));
@@ -406,8 +428,20 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
         int64_t smax = 0;
         for (size_t k : go) smax = std::max(smax, ps[k]->S());
         for (int64_t n = 0; n < smax; ++n) {
-            batch_fwd_rhs(s, dtab, smaps, nrhs, (int)n, first);
-            batch_fwd_gemm(s, dtab, smaps + nrhs, ngem, (int)n, sweep3m);
+            batch_fwd_rhs(s, dtab, smaps, nrhs, (int)n);
+            if (timer) timer->begin(s);
+            batch_fwd_gemm(s, dtab, smaps + nrhs, ngem, (int)n, sweep3m, first);
+            if (timer && timer->active) {
+                double fl = 0;  // executed flops of the launch: NP x m' x NP per problem
+                for (size_t k : go) {
+                    const HostProb& hp = *ps[k];
+                    if (n >= hp.S()) continue;
+                    const int64_t mc = (first && n < hp.S() - 1) ? std::min<int64_t>(hp.m, ceil_div(hp.mL, 4) * 4)
+                                                                  : hp.m;
+                    fl += fl3 * (double)hp.NP * (double)hp.NP * (double)mc;
+                }
+                timer->end(s, fl);
+            }
         }
         for (int64_t k = 0; k + 1 < smax; ++k) batch_bwd_gemm(s, dtab, smaps + nrhs, ngem, (int)k, sweep3m);
         batch_accumulate(s, dtab, smaps + nrhs + ngem, nacc, first);
@@ -429,6 +463,13 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
         RGF_HIP(hipStreamSynchronize(s));
     }
     const double t_refine = now() - t2;
+    double k_sec = 0, k_fl = 0;
+    int64_t k_n = 0;
+    if (timer) {
+        k_sec = timer->seconds();
+        k_fl = timer->flops;
+        k_n = timer->launches;
+    }
     for (size_t k : done) {
         HostProb& hp = *ps[k];
         const int64_t rows = hp.npo + hp.nqo, m = hp.m;
@@ -470,6 +511,10 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
             in.seconds_total = now() - t0;
             const double Nd = (double)hp.NP;
             in.flops_slices = (double)(hp.L + 1) * 8 * Nd * Nd * Nd * (use3m ? 0.75 : 1.0);
+            // the timed kernel is shared by the whole chunk
+            in.kernel_seconds = k_sec;
+            in.kernel_flops = k_fl;
+            in.kernel_launches = k_n;
         }
     }
 }
@@ -495,7 +540,7 @@ void solve_batched_engine(rgf_context* h, const std::vector<int32_t>& idx, const
         hp.user = q;
         hp.cplx_eps = p[q].eps_im != nullptr;
         const rgf_status st = guard(h, [&] {
-            hp.ch = setup_channels(&p[q]);
+            hp.ch = setup_channels(&p[q], false);  // mode tables: on the worker threads
             hp.sel = make_selection(&s[q], hp.ch);
             if (info) fill_info(&info[q], hp.ch, hp.sel);
             if (hp.sel.outL.size() + hp.sel.outR.size() == 0 || hp.sel.inL.size() + hp.sel.inR.size() == 0)

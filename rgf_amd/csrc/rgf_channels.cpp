@@ -31,35 +31,56 @@ cd kx_from_z(cd z) {
     return kx;
 }
 
-// One transverse direction: wavenumbers and the n x n mode table
-// f[m][a] = phi_a(m) (row-major), setup_channels.m:122-156.
-static void transverse_1d(int64_t n, int bc, double ka, std::vector<double>& k, std::vector<cd>& f) {
+// One transverse direction: the wavenumbers (setup_channels.m:122-156).
+static void transverse_k(int64_t n, int bc, double ka, std::vector<double>& k) {
     k.assign(n, 0.0);
-    f.assign((size_t)n * n, cd(0, 0));
     if (bc == RGF_BC_BLOCH) {
         const int64_t z = (n % 2 == 1) ? (n + 1) / 2 : n / 2;  // ind_zero_ky (:125-129)
         for (int64_t a = 0; a < n; ++a) k[a] = ka / n + (double)(a + 1 - z) * (2 * PI / n);
+    } else if (bc == RGF_BC_DIRICHLET) {
+        for (int64_t a = 0; a < n; ++a) k[a] = (double)(a + 1) * (PI / (n + 1));
+    } else if (bc == RGF_BC_NEUMANN) {
+        for (int64_t a = 0; a < n; ++a) k[a] = (double)a * (PI / n);
+    } else {  // DN / ND
+        for (int64_t a = 0; a < n; ++a) k[a] = (a + 0.5) * (PI / (n + 0.5));
+    }
+}
+
+// The n x n mode table f[m][a] = phi_a(m) (row-major) of one direction
+// (setup_channels.m:133-156).
+static void transverse_modes(int64_t n, int bc, const std::vector<double>& k, std::vector<cd>& f) {
+    f.assign((size_t)n * n, cd(0, 0));
+    if (bc == RGF_BC_BLOCH) {
         const double s = 1.0 / std::sqrt((double)n);
         for (int64_t m = 0; m < n; ++m)
             for (int64_t a = 0; a < n; ++a) f[m * n + a] = std::polar(s, (double)(m + 1) * k[a]);
     } else if (bc == RGF_BC_DIRICHLET) {
-        for (int64_t a = 0; a < n; ++a) k[a] = (double)(a + 1) * (PI / (n + 1));
         const double s = std::sqrt(2.0 / (n + 1));
         for (int64_t m = 0; m < n; ++m)
             for (int64_t a = 0; a < n; ++a) f[m * n + a] = std::sin((double)(m + 1) * k[a]) * s;
     } else if (bc == RGF_BC_NEUMANN) {
-        for (int64_t a = 0; a < n; ++a) k[a] = (double)a * (PI / n);
         const double s = std::sqrt(2.0 / n);
         for (int64_t m = 0; m < n; ++m)
             for (int64_t a = 0; a < n; ++a)
                 f[m * n + a] = (std::cos((m + 0.5) * k[a]) - (k[a] == 0 ? (1 - std::sqrt(0.5)) : 0.0)) * s;
     } else {  // DN / ND
-        for (int64_t a = 0; a < n; ++a) k[a] = (a + 0.5) * (PI / (n + 0.5));
         const double s = std::sqrt(2.0 / (n + 0.5));
         for (int64_t m = 0; m < n; ++m)
             for (int64_t a = 0; a < n; ++a)
                 f[m * n + a] = (bc == RGF_BC_DIRICHLET_NEUMANN) ? std::sin((double)(m + 1) * k[a]) * s
                                                                 : std::cos((m + 0.5) * k[a]) * s;
+    }
+}
+
+void build_modes(Channels& ch) {
+    std::vector<double> k;
+    transverse_k(ch.n1, ch.bc[0], ch.ka[0], k);
+    transverse_modes(ch.n1, ch.bc[0], k, ch.f1);
+    if (ch.ndim == 3) {
+        transverse_k(ch.n2, ch.bc[1], ch.ka[1], k);
+        transverse_modes(ch.n2, ch.bc[1], k, ch.f2);
+    } else {
+        ch.f2.assign(1, cd(1, 0));
     }
 }
 
@@ -130,7 +151,7 @@ void validate_problem(const rgf_problem* p) {
     if (total > 0 && !p->eps_re) throw Error(RGF_ERR_INVALID_ARGUMENT, "syst.epsilon must be provided");
 }
 
-Channels setup_channels(const rgf_problem* p) {
+Channels setup_channels(const rgf_problem* p, bool modes) {
     validate_problem(p);
     Channels ch;
     ch.ndim = p->ndim;
@@ -153,13 +174,12 @@ Channels setup_channels(const rgf_problem* p) {
     }
     ch.N = ch.n1 * ch.n2;
     std::vector<double> k1, k2;
-    transverse_1d(ch.n1, ch.bc[0], ch.ka[0], k1, ch.f1);
-    if (ch.ndim == 3) {
-        transverse_1d(ch.n2, ch.bc[1], ch.ka[1], k2, ch.f2);
-    } else {
+    transverse_k(ch.n1, ch.bc[0], ch.ka[0], k1);
+    if (ch.ndim == 3)
+        transverse_k(ch.n2, ch.bc[1], ch.ka[1], k2);
+    else
         k2.assign(1, 0.0);
-        ch.f2.assign(1, cd(1, 0));
-    }
+    if (modes) build_modes(ch);
     ch.kt1.resize(ch.N);
     ch.kt2.resize(ch.N);
     std::vector<double> sin2(ch.N);

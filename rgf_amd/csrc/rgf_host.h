@@ -44,7 +44,11 @@ struct Selection {                  // 0-based channel indices, user order
 
 cd kx_from_z(cd z);
 void validate_problem(const rgf_problem* p);
-Channels setup_channels(const rgf_problem* p);
+// modes = false leaves the transverse mode tables f1, f2 empty (the channel
+// counts, wavenumbers and lead dispersion do not need them); build_modes
+// fills them later.
+Channels setup_channels(const rgf_problem* p, bool modes = true);
+void build_modes(Channels& ch);
 std::map<std::pair<int64_t, int64_t>, cd> laplacian_1d(int64_t n, int bc, double ka);
 HostCSR slice_stencil(const Channels& ch);
 Selection make_selection(const rgf_selection* s, const Channels& ch);

@@ -210,24 +210,31 @@ struct BatchProb {
 };
 
 // Work maps: entry k of a map is (problem << 20) | local index.
-// Dyson rounds (round r read as *rbase + k): k1 = diagonal-block inverses
-// (work groups [0, np)) and the write-back / assembly chunks (k1map, n1);
-// k2 = panel products (k2map, n2 tiles); k3 = trailing updates (k3map, n3).
-void batch_round(hipStream_t s, const BatchProb* P, int np, const int* rbase, int k, const int32_t* k1map, int n1,
-                 const int32_t* k2map, int n2, const int32_t* k3map, int n3, int use3m);
+// Dyson round r (read as *rbase + k): the diagonal-block inverses (one work
+// group per table entry) on s beside the write-back / assembly row pass
+// (k1map: groups of kBatchRows rows, n1) on s2 (forked and joined with the
+// two events), then the panel products (k2map, n2 tiles) and the trailing
+// updates (k3map, n3 tiles) on s.
+void batch_round(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join, const BatchProb* P, int np,
+                 const int* rbase, int k, const int32_t* k1map, int n1, const int32_t* k2map, int n2,
+                 const int32_t* k3map, int n3, int use3m);
+constexpr int kBatchRows = 4;  // rows per work group of the row-wise passes (a wavefront per row)
 // Refinement.  map entries index the sweep-product tiles (NP/64 x m/64 per
-// problem) or the 16384-element chunks of an elementwise pass.
-// residual maximum: first = psi is zero (only the lead slices); chunks over
-// the slices' N x m entries.
+// problem), the row groups of the residual pass or the kBatchChunk-element
+// chunks of an elementwise pass.
+// residual pass: r = b - H psi in FP32 into z, max |r| per problem into
+// rmax; first = psi is zero (only the lead slices, groups of 64 rows of the
+// 2 N; otherwise of the S N slice rows)
 void batch_resmax(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first);
+constexpr int kBatchResRows = 64;  // slice rows per work group of the residual pass
 // lead terms W = G0 psi at the two lead slices (FP64 products)
 void batch_lead_terms(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles);
-// forward step n: t = (float)(r_n / rmax) + z_{n-1}; then z_n = g_n t
-void batch_fwd_rhs(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int n, int first);
-void batch_fwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int n, int use3m);
+// forward step n: t = r_n + z_{n-1} (chunks of N m); then z_n = g_n t
+void batch_fwd_rhs(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int n);
+void batch_fwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int n, int use3m, int first);
 // backward step k: z_n += g_n z_{n+1} with n = L - k (each problem's own end)
 void batch_bwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int k, int use3m);
-// psi += rmax z over all slices (first: psi = rmax z)
+// psi += z over all slices (first: psi = z)
 void batch_accumulate(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first);
 // T = [C_L psi_0; C_R psi_{L+1}]
 void batch_project(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles);

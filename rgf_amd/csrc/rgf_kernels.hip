@@ -2691,104 +2691,171 @@ __device__ __forceinline__ void slice_entry(const BatchProb& pb, int n, int i, i
     outi = (float)vi;
 }
 
-// Round r, first launch.  Work groups [0, np): the diagonal-block inverse of
-// problem w's step r (slice n = 1 + r / nb, block q = r % nb; at q = 0 the
-// group first assembles that block of the slice matrix).  The others: chunks
-// of the pending write-back of step r - 1 (rows and columns of its block
-// from R and C) and, at q = 0, the assembly of the rest of slice n (which
-// folds in the write-back of slice n - 1's last block).  Step (L + 1) nb
-// only writes back.
-__global__ __launch_bounds__(256) void batch_k1_kernel(const BatchProb* P, int np, const int* rbase, int k,
-                                                       const int32_t* map1) {
+// Rows of a slice handled by one work group of the row-wise passes (four
+// wavefronts, four rows each: a wavefront takes a row, its lanes the columns).
+constexpr int kRowsPerGroup = kBatchRows;
+static_assert(kRowsPerGroup == 4, "a wavefront per row, four per work group");
+
+// Round r, diagonal blocks: work group w inverts the diagonal block of
+// problem w's step r (slice n = 1 + r / nb, block q = r % nb); at q = 0 it
+// first assembles that block of the slice matrix (for nb > 1 the block lies
+// outside the pending write-back's rows and columns; for nb = 1 nothing is
+// pending).  Runs beside batch_rows_kernel of the same round.
+__global__ __launch_bounds__(256) void batch_diag_kernel(const BatchProb* P, const int* rbase, int k) {
     const int r = *rbase + k;
-    const int w = (int)blockIdx.x;
-    const int tid = (int)threadIdx.x;
-    if (w < np) {
-        const BatchProb& pb = P[w];
-        const int nb = pb.nb, NP = pb.NP;
-        if (r >= (pb.L + 1) * nb) return;
-        const int n = 1 + r / nb, q = r % nb;
-        const size_t NN = (size_t)NP * NP;
-        float* Fr = pb.F + (size_t)n * 2 * NN;
-        float* Fi = Fr + NN;
-        if (q == 0) {
-            // block (0, 0) of slice n; for nb > 1 it lies outside the pending
-            // write-back's rows and columns, for nb = 1 nothing is pending
-            const float* Pr = Fr - 2 * NN;
-            const float* Pi = Pr + NN;
-            for (int e = tid; e < 64 * 64; e += 256) {
-                const int i = e >> 6, j = e & 63;
-                float gr, gi, vr, vi;
-                prev_entry(pb, Pr, Pi, false, 0, i, j, gr, gi);
-                slice_entry(pb, n, i, j, gr, gi, vr, vi);
-                Fr[(size_t)i * NP + j] = vr;
-                Fi[(size_t)i * NP + j] = vi;
-            }
-            __syncthreads();
-        }
-        gj_diag_blk<float, 64>(Fr, Fi, NP, q * 64, 64, ws_D(pb), ws_D(pb) + 64 * 64, 64, pb.status);
-        return;
-    }
-    const int32_t e0 = map1[w - np];
-    const BatchProb& pb = P[map_prob(e0)];
-    const int c = map_loc(e0);
-    const int nb = pb.nb, NP = pb.NP, T = (pb.L + 1) * nb;
-    if (r > T) return;
-    const bool asmb = r < T && (r % nb) == 0;
-    const bool pending = r >= 1 && nb > 1;
-    const int64_t count = asmb ? (int64_t)NP * NP : (pending ? (int64_t)128 * (NP - 64) : 0);
-    const int64_t lo = (int64_t)c * kBatchChunk;
-    if (lo >= count) return;
-    const int64_t hi = min(lo + (int64_t)kBatchChunk, count);
+    const BatchProb& pb = P[blockIdx.x];
+    const int nb = pb.nb, NP = pb.NP;
+    if (r >= (pb.L + 1) * nb) return;
+    const int n = 1 + r / nb, q = r % nb;
     const size_t NN = (size_t)NP * NP;
-    const int ps = r - 1;                            // the pending step
-    const int p0w = pending ? (ps % nb) * 64 : 0;
-    float* Wr = pb.F + (size_t)(pending ? 1 + ps / nb : 0) * 2 * NN;  // its slice
-    float* Wi = Wr + NN;
-    if (asmb) {
-        const int n = 1 + r / nb;
-        float* Fr = pb.F + (size_t)n * 2 * NN;
-        float* Fi = Fr + NN;
-        const float* Pr = Fr - 2 * NN;  // slice n - 1 (= the pending slice when pending)
+    float* Fr = pb.F + (size_t)n * 2 * NN;
+    float* Fi = Fr + NN;
+    if (q == 0) {
+        const float* Pr = Fr - 2 * NN;
         const float* Pi = Pr + NN;
-        for (int64_t e = lo + tid; e < hi; e += 256) {
-            const int i = (int)(e / NP), j = (int)(e % NP);
-            if (i < 64 && j < 64) continue;  // the diagonal group's block
+        for (int e = (int)threadIdx.x; e < 64 * 64; e += 256) {
+            const int i = e >> 6, j = e & 63;
             float gr, gi, vr, vi;
-            prev_entry(pb, Pr, Pi, pending, p0w, i, j, gr, gi);
-            const bool ri = i >= p0w && i < p0w + 64, cj = j >= p0w && j < p0w + 64;
-            if (pending && ri != cj) {
-                Wr[(size_t)i * NP + j] = gr;
-                Wi[(size_t)i * NP + j] = gi;
-            }
+            prev_entry(pb, Pr, Pi, false, 0, i, j, gr, gi);
             slice_entry(pb, n, i, j, gr, gi, vr, vi);
             Fr[(size_t)i * NP + j] = vr;
             Fi[(size_t)i * NP + j] = vi;
         }
-        return;
+        __syncthreads();
     }
+    gj_diag_blk<float, 64>(Fr, Fi, NP, q * 64, 64, ws_D(pb), ws_D(pb) + 64 * 64, 64, pb.status);
+}
+
+// Round r, row pass (beside the diagonal blocks): the pending write-back of
+// step r - 1 (its block's rows from R, its block's columns from C) and, when
+// step r starts a slice (q = 0), the assembly of that slice, which folds in
+// the write-back of the previous slice's last block.  The diagonal group owns
+// block (0, 0) of an assembled slice.  Step (L + 1) nb only writes back.
+// A wavefront per row, four consecutive columns per lane (16-byte loads and
+// stores; a lane's four columns lie in one 64-wide block).
+// map: (problem, group of kRowsPerGroup rows).
+__global__ __launch_bounds__(256) void batch_rows_kernel(const BatchProb* P, const int* rbase, int k,
+                                                         const int32_t* map) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int r = *rbase + k;
+    const int32_t e0 = map[blockIdx.x];
+    const BatchProb& pb = P[map_prob(e0)];
+    const int nb = pb.nb, NP = pb.NP, N = pb.N, T = (pb.L + 1) * nb;
+    if (r > T) return;
+    const bool asmb = r < T && (r % nb) == 0;
+    const bool pending = r >= 1 && nb > 1;
+    if (!asmb && !pending) return;
+    const size_t NN = (size_t)NP * NP;
+    const int ps = r - 1;
+    const int p0w = pending ? (ps % nb) * 64 : 0;
+    float* Wr = pb.F + (size_t)(pending ? 1 + ps / nb : 0) * 2 * NN;  // the pending step's slice
+    float* Wi = Wr + NN;
     const float* R = ws_R(pb);
     const float* C = ws_C(pb);
-    const int64_t half = (int64_t)64 * (NP - 64);
-    for (int64_t e = lo + tid; e < hi; e += 256) {
-        int i, j;
-        float vr, vi;
-        if (e < half) {
-            const int a = (int)(e / (NP - 64)), jj = (int)(e % (NP - 64));
-            i = p0w + a;
-            j = jj < p0w ? jj : jj + 64;
-            vr = R[(size_t)a * NP + j];
-            vi = R[(size_t)64 * NP + (size_t)a * NP + j];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    const int i = map_loc(e0) * kRowsPerGroup + wave;
+    if (i >= NP) return;
+    const bool rowP = pending && i >= p0w && i < p0w + 64;
+    if (!asmb) {
+        if (rowP) {
+            const size_t ro = (size_t)(i - p0w) * NP;
+            for (int j = 4 * lane; j < NP; j += 256) {
+                if (j >= p0w && j < p0w + 64) continue;
+                *(f4*)(Wr + (size_t)i * NP + j) = *(const f4*)(R + ro + j);
+                *(f4*)(Wi + (size_t)i * NP + j) = *(const f4*)(R + (size_t)64 * NP + ro + j);
+            }
         } else {
-            const int64_t e2 = e - half;
-            const int ii = (int)(e2 >> 6), b = (int)(e2 & 63);
-            i = ii < p0w ? ii : ii + 64;
-            j = p0w + b;
-            vr = C[(size_t)i * 64 + b];
-            vi = C[(size_t)NP * 64 + (size_t)i * 64 + b];
+            Wr[(size_t)i * NP + p0w + lane] = C[(size_t)i * 64 + lane];
+            Wi[(size_t)i * NP + p0w + lane] = C[(size_t)NP * 64 + (size_t)i * 64 + lane];
         }
-        Wr[(size_t)i * NP + j] = vr;
-        Wi[(size_t)i * NP + j] = vi;
+        return;
+    }
+    // row i of slice n: A_n(i, :) - g_{n-1}(i, :)
+    const int n = 1 + r / nb;
+    float* Fr = pb.F + (size_t)n * 2 * NN;
+    float* Fi = Fr + NN;
+    const float* Pr = Fr - 2 * NN;
+    const float* Pi = Pr + NN;
+    const bool real_row = i < N;
+    double dr = 0.0, di = 0.0;  // -k^2 eps_i on the diagonal
+    int rp0 = 0, rp1 = 0;
+    if (real_row) {
+        double er, ei;
+        if (n <= pb.L) {
+            er = pb.eps_re[(size_t)(n - 1) * N + i];
+            ei = pb.eps_im ? pb.eps_im[(size_t)(n - 1) * N + i] : 0.0;
+        } else {
+            er = pb.eRr;
+            ei = pb.eRi;
+        }
+        dr = -(pb.k2r * er - pb.k2i * ei);
+        di = -(pb.k2r * ei + pb.k2i * er);
+        rp0 = pb.rowptr[i];
+        rp1 = pb.rowptr[i + 1];
+    }
+    const bool lead = n == pb.L + 1;
+    for (int j = 4 * lane; j < NP; j += 256) {
+        if (i < 64 && j < 64) continue;
+        const bool colP = j >= p0w && j < p0w + 64;
+        f4 gr, gi;
+        if (pending && rowP && !colP) {
+            const size_t ro = (size_t)(i - p0w) * NP + j;
+            gr = *(const f4*)(R + ro);
+            gi = *(const f4*)(R + (size_t)64 * NP + ro);
+        } else if (pending && !rowP && colP) {
+            const size_t co = (size_t)i * 64 + (j - p0w);
+            gr = *(const f4*)(C + co);
+            gi = *(const f4*)(C + (size_t)NP * 64 + co);
+        } else {
+            gr = *(const f4*)(Pr + (size_t)i * NP + j);
+            gi = *(const f4*)(Pi + (size_t)i * NP + j);
+        }
+        if (pending && rowP != colP) {
+            *(f4*)(Wr + (size_t)i * NP + j) = gr;
+            *(f4*)(Wi + (size_t)i * NP + j) = gi;
+        }
+        f4 vr, vi;
+        if (!real_row || j >= N) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                vr[u] = (i == j + u) ? 1.f : 0.f;
+                vi[u] = 0.f;
+            }
+        } else {
+            double ar[4], ai[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ar[u] = -(double)gr[u];
+                ai[u] = -(double)gi[u];
+            }
+            if (lead) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const size_t o = (size_t)i * NP + j + u;
+                    ar[u] -= pb.G0R[o];
+                    ai[u] -= pb.G0R[NN + o];
+                }
+            }
+            if (i >= j && i < j + 4) {
+                ar[i - j] += dr;
+                ai[i - j] += di;
+            }
+            for (int q = rp0; q < rp1; ++q) {
+                const int cq = pb.col[q] - j;
+                if (cq >= 0 && cq < 4) {
+                    ar[cq] += pb.vre[q];
+                    ai[cq] += pb.vim[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                // columns j + u >= N (the padding) of a real row are zero
+                vr[u] = (j + u < N) ? (float)ar[u] : 0.f;
+                vi[u] = (j + u < N) ? (float)ai[u] : 0.f;
+            }
+        }
+        *(f4*)(Fr + (size_t)i * NP + j) = vr;
+        *(f4*)(Fi + (size_t)i * NP + j) = vi;
     }
 }
 
@@ -2881,21 +2948,15 @@ __global__ __launch_bounds__(256) void batch_first_slice_kernel(const BatchProb*
 // b holds B_L at slice 0 (columns < mL) and B_R at slice L + 1 (columns
 // [mL, mL + mR)); H psi = (A0 - k^2 eps_n) psi_n - psi_{n-1} - psi_{n+1},
 // with the lead terms G0L psi_0 and G0R psi_{L+1} (W) at the end slices.
-// zero_psi: psi = 0 (the first correction), r = b.
-__device__ __forceinline__ void batch_residual(const BatchProb& pb, int n, int i, int c, bool zero_psi, double& rr,
-                                               double& ri) {
-    const int m = pb.m, S = pb.L + 2;
-    const size_t SE = (size_t)pb.NP * m;
-    const size_t o = (size_t)i * m + c;
-    rr = 0.0;
-    ri = 0.0;
-    if ((n == 0 && c < pb.mL) || (n == S - 1 && c >= pb.mL && c < pb.mL + pb.mR)) {
-        rr = pb.B[o];
-        ri = pb.B[SE + o];
-    }
-    if (zero_psi) return;
-    const double* pr = pb.psi;
-    const double* pim = pb.psi + (size_t)S * SE;
+// zero_psi: psi = 0 (the first correction), r = b.  The per-row values
+// (k^2 eps_n(i) and the stencil row) come from ResRow.
+struct ResRow {
+    double ker, kei;
+    int rp0, rp1;
+};
+
+__device__ __forceinline__ ResRow res_row(const BatchProb& pb, int n, int i) {
+    const int S = pb.L + 2;
     double er, ei;
     if (n == 0) {
         er = pb.eLr;
@@ -2907,23 +2968,43 @@ __device__ __forceinline__ void batch_residual(const BatchProb& pb, int n, int i
         er = pb.eps_re[(size_t)(n - 1) * pb.N + i];
         ei = pb.eps_im ? pb.eps_im[(size_t)(n - 1) * pb.N + i] : 0.0;
     }
-    const double ker = pb.k2r * er - pb.k2i * ei, kei = pb.k2r * ei + pb.k2i * er;
-    const size_t e = (size_t)n * SE + o;
-    const double psr = pr[e], psi = pim[e];
-    double ar = ker * psr - kei * psi, ai = ker * psi + kei * psr;
-    for (int q = pb.rowptr[i]; q < pb.rowptr[i + 1]; ++q) {
-        const size_t oq = (size_t)n * SE + (size_t)pb.col[q] * m + c;
+    ResRow rw;
+    rw.ker = pb.k2r * er - pb.k2i * ei;
+    rw.kei = pb.k2r * ei + pb.k2i * er;
+    rw.rp0 = pb.rowptr[i];
+    rw.rp1 = pb.rowptr[i + 1];
+    return rw;
+}
+
+__device__ __forceinline__ void batch_residual(const BatchProb& pb, const ResRow& rw, int n, int i, int c,
+                                               bool zero_psi, double& rr, double& ri) {
+    const int m = pb.m, S = pb.L + 2;
+    const size_t SE = (size_t)pb.NP * m;
+    const size_t o = (size_t)i * m + c;
+    rr = 0.0;
+    ri = 0.0;
+    if ((n == 0 && c < pb.mL) || (n == S - 1 && c >= pb.mL && c < pb.mL + pb.mR)) {
+        rr = pb.B[o];
+        ri = pb.B[SE + o];
+    }
+    if (zero_psi) return;
+    const double* pr = pb.psi + (size_t)n * SE;
+    const double* pim = pb.psi + (size_t)S * SE + (size_t)n * SE;
+    const double psr = pr[o], psi = pim[o];
+    double ar = rw.ker * psr - rw.kei * psi, ai = rw.ker * psi + rw.kei * psr;
+    for (int q = rw.rp0; q < rw.rp1; ++q) {
+        const size_t oq = (size_t)pb.col[q] * m + c;
         const double vr = pb.vre[q], vi = pb.vim[q];
         ar -= vr * pr[oq] - vi * pim[oq];
         ai -= vr * pim[oq] + vi * pr[oq];
     }
     if (n > 0) {
-        ar += pr[e - SE];
-        ai += pim[e - SE];
+        ar += pr[o - SE];
+        ai += pim[o - SE];
     }
     if (n < S - 1) {
-        ar += pr[e + SE];
-        ai += pim[e + SE];
+        ar += pr[o + SE];
+        ai += pim[o + SE];
     }
     if (n == 0) {
         ar += pb.W[o];
@@ -2937,73 +3018,94 @@ __device__ __forceinline__ void batch_residual(const BatchProb& pb, int n, int i
     ri += ai;
 }
 
-// max |r| per problem over its slices (first: psi = 0, the lead slices only)
+// Residual pass: r = b - H psi over every slice (first: psi = 0, only the
+// lead slices, where b lives; the other slices of z are zero), stored in
+// FP32 into z (the previous correction is already in psi; z_n holds r_n
+// until the forward sweep replaces it by g_n (r_n + z_{n-1})), and max |r|
+// per problem.  A wavefront per row, kResRowsPerWave rows each; map:
+// (problem, group of 4 kResRowsPerWave rows of the S N (first: 2 N) rows).
+constexpr int kResRowsPerWave = kBatchResRows / 4;
 __global__ __launch_bounds__(256) void batch_resmax_kernel(const BatchProb* P, const int32_t* map, int first) {
     const int32_t e0 = map[blockIdx.x];
     const BatchProb& pb = P[map_prob(e0)];
-    const int64_t per = (int64_t)pb.N * pb.m;
-    const int S = pb.L + 2;
-    const int64_t count = first ? 2 * per : (int64_t)S * per;
-    const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, count);
+    const int N = pb.N, S = pb.L + 2, m = pb.m;
+    const int rows = first ? 2 * N : S * N;
+    const size_t SE = (size_t)pb.NP * m;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     double mx = 0.0;
-    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
-        const int64_t sl = e / per, rem = e - sl * per;
-        const int n = first ? (sl ? S - 1 : 0) : (int)sl;
-        const int i = (int)(rem / pb.m), c = (int)(rem % pb.m);
-        double rr, ri;
-        batch_residual(pb, n, i, c, first != 0, rr, ri);
-        const double v = sqrt(rr * rr + ri * ri);
-        if (!(v <= mx)) mx = v;  // NaN propagates
+    for (int rr = 0; rr < kResRowsPerWave; ++rr) {
+        const int u = (map_loc(e0) * 4 + wave) * kResRowsPerWave + rr;
+        if (u >= rows) break;
+        const int sl = u / N, i = u - sl * N;
+        const int n = first ? (sl ? S - 1 : 0) : sl;
+        const ResRow rw = res_row(pb, n, i);
+        float* zr = pb.z + (size_t)n * SE + (size_t)i * m;
+        float* zi = pb.z + (size_t)S * SE + (size_t)n * SE + (size_t)i * m;
+        for (int c = lane; c < m; c += 64) {
+            double rr_, ri_;
+            batch_residual(pb, rw, n, i, c, first != 0, rr_, ri_);
+            zr[c] = (float)rr_;
+            zi[c] = (float)ri_;
+            const double v = sqrt(rr_ * rr_ + ri_ * ri_);
+            if (!(v <= mx)) mx = v;  // NaN propagates
+        }
     }
     for (int off = 32; off >= 1; off >>= 1) {
         const double o = __shfl_xor(mx, off);
         if (!(o <= mx)) mx = o;
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         const unsigned long long bits =
             isnan(mx) ? 0x7ff8000000000000ULL : (unsigned long long)__double_as_longlong(mx);
         atomicMax(pb.rmax, bits);
     }
 }
 
-// forward step n: t = (float)(r_n / rmax) + z_{n-1} (t's padding rows stay zero)
-__global__ __launch_bounds__(256) void batch_fwd_rhs_kernel(const BatchProb* P, const int32_t* map, int n, int first) {
+// forward step n: t = r_n + z_{n-1} (r_n in z_n, z_{n-1} = g_{n-1} t_{n-1}
+// already), over the N x m real rows (the padding rows stay zero); map:
+// (problem, chunk of kBatchChunk elements of N m, both planes), 16-byte loads.
+__global__ __launch_bounds__(256) void batch_fwd_rhs_kernel(const BatchProb* P, const int32_t* map, int n) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const int32_t e0 = map[blockIdx.x];
     const BatchProb& pb = P[map_prob(e0)];
     const int S = pb.L + 2;
     if (n >= S) return;
-    const int64_t per = (int64_t)pb.N * pb.m;
-    const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, per);
-    const double sc = 1.0 / __longlong_as_double((long long)*pb.rmax);
+    const int tot = pb.N * pb.m;  // a multiple of 4 (m is)
     const size_t SE = (size_t)pb.NP * pb.m;
-    float* tr = pb.t;
-    float* ti = pb.t + SE;
-    const float* zr = pb.z + (size_t)(n - 1) * SE;
-    const float* zi = pb.z + (size_t)S * SE + (size_t)(n - 1) * SE;
-    for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
-        const int i = (int)(e / pb.m), c = (int)(e % pb.m);
-        double rr, ri;
-        batch_residual(pb, n, i, c, first != 0, rr, ri);
-        float vr = (float)(rr * sc), vi = (float)(ri * sc);
+    const float* rr = pb.z + (size_t)n * SE;
+    const float* ri = pb.z + (size_t)S * SE + (size_t)n * SE;
+    const int lo = map_loc(e0) * kBatchChunk, hi = min(lo + kBatchChunk, tot);
+    for (int e = lo + 4 * (int)threadIdx.x; e < hi; e += 1024) {
+        f4 vr = *(const f4*)(rr + e), vi = *(const f4*)(ri + e);
         if (n > 0) {
-            vr += zr[e];
-            vi += zi[e];
+            vr += *(const f4*)(rr - SE + e);
+            vi += *(const f4*)(ri - SE + e);
         }
-        tr[e] = vr;
-        ti[e] = vi;
+        *(f4*)(pb.t + e) = vr;
+        *(f4*)(pb.t + SE + e) = vi;
     }
 }
 
 // z_n = g_n t (forward) or z_n += g_n z_{n+1} with n = L - k (backward):
-// NP x m products, K = NP, tiles NP/64 x ceil(m/64)
+// NP x m products, K = NP, tiles NP/64 x ceil(m/64).  first (forward, the
+// first correction): before the last slice only the left inputs' columns
+// [0, ceil4(mL)) are non-zero (the right-hand side is B_L at slice 0), so the
+// products run on those columns; z starts zeroed.
 template <bool M3, bool BWD>
-__global__ __launch_bounds__(256, 4) void batch_sweep_gemm_kernel(const BatchProb* P, const int32_t* map, int step) {
+__global__ __launch_bounds__(256, 4) void batch_sweep_gemm_kernel(const BatchProb* P, const int32_t* map, int step,
+                                                                  int first) {
     const int32_t e0 = map[blockIdx.x];
     const BatchProb& pb = P[map_prob(e0)];
-    const int S = pb.L + 2, NP = pb.NP, m = pb.m;
+    const int S = pb.L + 2, NP = pb.NP;
     const int n = BWD ? S - 2 - step : step;
     if (n < 0 || n >= S) return;
-    const size_t NN = (size_t)NP * NP, SE = (size_t)NP * m;
+    int m = pb.m;
+    const int tn = (m + 63) / 64;
+    if (!BWD && first && n < S - 1) {
+        m = min(m, (pb.mL + 3) / 4 * 4);
+        if ((map_loc(e0) % tn) * 64 >= m) return;
+    }
+    const size_t NN = (size_t)NP * NP, SE = (size_t)NP * pb.m;
     const float* Fr = pb.F + (size_t)n * 2 * NN;
     const float* Fi = Fr + NN;
     float* zr = pb.z + (size_t)n * SE;
@@ -3011,7 +3113,7 @@ __global__ __launch_bounds__(256, 4) void batch_sweep_gemm_kernel(const BatchPro
     const float* br = BWD ? zr + SE : pb.t;
     const float* bi = BWD ? zi + SE : pb.t + SE;
     const GemmArgs<float> g =
-        dev_args<float>(NP, m, NP, Fr, Fi, NP, br, bi, m, zr, zi, m, 1.0, BWD ? 1 : 0, (m + 63) / 64);
+        dev_args<float>(NP, m, NP, Fr, Fi, NP, br, bi, pb.m, zr, zi, pb.m, 1.0, BWD ? 1 : 0, tn);
     cgemm2_tile<float, 64, 64, 16, 2, 2, false, 0, true, M3, false>(g, map_loc(e0));
 }
 
@@ -3053,24 +3155,30 @@ __global__ __launch_bounds__(256) void batch_project_kernel(const BatchProb* P, 
     cgemm2_tile<double, 64, 64, 16, 2, 2, false, 0, true, false, false>(g, t);
 }
 
-// psi += rmax z over every slice (first: psi = rmax z, psi was never written)
+// psi += z over every slice (first: psi = z, psi was never written)
 __global__ __launch_bounds__(256) void batch_accumulate_kernel(const BatchProb* P, const int32_t* map, int first) {
     const int32_t e0 = map[blockIdx.x];
     const BatchProb& pb = P[map_prob(e0)];
     const int64_t tot = (int64_t)(pb.L + 2) * pb.NP * pb.m;
     const int64_t lo = (int64_t)map_loc(e0) * kBatchChunk, hi = min(lo + (int64_t)kBatchChunk, tot);
-    const double sc = __longlong_as_double((long long)*pb.rmax);
     for (int64_t e = lo + threadIdx.x; e < hi; e += 256) {
-        const double a = (double)pb.z[e] * sc, b = (double)pb.z[tot + e] * sc;
+        const double a = (double)pb.z[e], b = (double)pb.z[tot + e];
         pb.psi[e] = first ? a : pb.psi[e] + a;
         pb.psi[tot + e] = first ? b : pb.psi[tot + e] + b;
     }
 }
 
 // ------------------------------------------------------------------ launchers
-void batch_round(hipStream_t s, const BatchProb* P, int np, const int* rbase, int k, const int32_t* k1map, int n1,
-                 const int32_t* k2map, int n2, const int32_t* k3map, int n3, int use3m) {
-    hipLaunchKernelGGL(batch_k1_kernel, dim3((unsigned)(np + n1)), dim3(256), 0, s, P, np, rbase, k, k1map);
+void batch_round(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join, const BatchProb* P, int np,
+                 const int* rbase, int k, const int32_t* k1map, int n1, const int32_t* k2map, int n2,
+                 const int32_t* k3map, int n3, int use3m) {
+    // the row pass on the second stream, beside the diagonal blocks
+    RGF_HIP(hipEventRecord(fork, s));
+    RGF_HIP(hipStreamWaitEvent(s2, fork, 0));
+    if (n1 > 0) hipLaunchKernelGGL(batch_rows_kernel, dim3((unsigned)n1), dim3(256), 0, s2, P, rbase, k, k1map);
+    RGF_HIP(hipEventRecord(join, s2));
+    hipLaunchKernelGGL(batch_diag_kernel, dim3((unsigned)np), dim3(256), 0, s, P, rbase, k);
+    RGF_HIP(hipStreamWaitEvent(s, join, 0));
     if (n2 > 0) {
         if (use3m)
             hipLaunchKernelGGL((batch_k2_kernel<true>), dim3((unsigned)n2), dim3(256), 0, s, P, rbase, k, k2map);
@@ -3108,26 +3216,29 @@ void batch_lead_terms(hipStream_t s, const BatchProb* P, const int32_t* map, int
     RGF_HIP(hipGetLastError());
 }
 
-void batch_fwd_rhs(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int n, int first) {
+void batch_fwd_rhs(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int n) {
     if (nchunks <= 0) return;
-    hipLaunchKernelGGL(batch_fwd_rhs_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, P, map, n, first);
+    hipLaunchKernelGGL(batch_fwd_rhs_kernel, dim3((unsigned)nchunks), dim3(256), 0, s, P, map, n);
 }
 
-void batch_fwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int n, int use3m) {
+void batch_fwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int n, int use3m, int first) {
     if (ntiles <= 0) return;
     if (use3m)
-        hipLaunchKernelGGL((batch_sweep_gemm_kernel<true, false>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, n);
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<true, false>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, n,
+                           first);
     else
         hipLaunchKernelGGL((batch_sweep_gemm_kernel<false, false>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map,
-                           n);
+                           n, first);
 }
 
 void batch_bwd_gemm(hipStream_t s, const BatchProb* P, const int32_t* map, int ntiles, int k, int use3m) {
     if (ntiles <= 0) return;
     if (use3m)
-        hipLaunchKernelGGL((batch_sweep_gemm_kernel<true, true>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, k);
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<true, true>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, k,
+                           0);
     else
-        hipLaunchKernelGGL((batch_sweep_gemm_kernel<false, true>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, k);
+        hipLaunchKernelGGL((batch_sweep_gemm_kernel<false, true>), dim3((unsigned)ntiles), dim3(256), 0, s, P, map, k,
+                           0);
 }
 
 void batch_accumulate(hipStream_t s, const BatchProb* P, const int32_t* map, int nchunks, int first) {

@@ -2,9 +2,9 @@
 and the batch_* kernels of rgf_kernels.hip).
 
 The model repeats the kernels' index arithmetic element by element (round ->
-slice and block, the write-back mapping of block rows and columns, the
-chunking of the elementwise passes, the tiles skipped around the pivot block,
-the backward sweep's per-problem slice) with explicit bounds checks, over a
+slice and block, the row pass's write-back of block rows and columns and its
+assembly of the next slice, the tiles skipped around the pivot block, the
+backward sweep's per-problem slice) with explicit bounds checks, over a
 ragged batch, and checks the result against the slice recursion done
 directly: every stored slice inverse, and the solution of the whole
 block-tridiagonal system after one forward/backward sweep through exact
@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 BLK = 64
+ROWS = 16  # rows per work group of the row-wise passes
 
 
 def _rand_problem(rng, N, L):
@@ -80,10 +81,9 @@ def _prev_entry(pb, Prv, pending, p0w, i, j):
     return Prv[i, j]
 
 
-def k1(probs, r, CH):
-    # diagonal groups first (they run beside the chunks; the chunks never touch
-    # what the diagonal groups read or write, which the model checks by
-    # running the diagonal groups on a snapshot)
+def k1(probs, r):
+    # diagonal groups first (they run beside the row pass, which never touches
+    # what they read or write: the model runs them on a snapshot)
     snap = [pb.F.copy() for pb in probs]
     for w, pb in enumerate(probs):
         nb, NP = pb.nb, pb.NP
@@ -107,42 +107,40 @@ def k1(probs, r, CH):
             continue
         asmb = r < T and r % nb == 0
         pending = r >= 1 and nb > 1
-        count = NP * NP if asmb else (128 * (NP - 64) if pending else 0)
+        if not asmb and not pending:
+            continue
         ps = r - 1
         p0w = (ps % nb) * BLK if pending else 0
         Wsl = 1 + ps // nb if pending else 0
-        for c in range(-(-NP * NP // CH)):
-            lo = c * CH
-            if lo >= count:
-                continue
-            hi = min(lo + CH, count)
-            for e in range(lo, hi):
-                if asmb:
-                    n = 1 + r // nb
-                    i, j = e // NP, e % NP
+        n = 1 + r // nb
+        for g in range(NP // ROWS):               # batch_rows_kernel: groups of rows
+            for i in range(g * ROWS, (g + 1) * ROWS):
+                _chk((i, NP))
+                rowP = pending and p0w <= i < p0w + BLK
+                if not asmb:
+                    if rowP:
+                        for j in range(NP):
+                            if p0w <= j < p0w + BLK:
+                                continue
+                            v = pb.R[i - p0w, j]
+                            assert not np.isnan(v)
+                            pb.F[Wsl][i, j] = v
+                    else:
+                        for lane in range(BLK):
+                            j = p0w + lane
+                            v = pb.C[i, lane]
+                            assert not np.isnan(v)
+                            pb.F[Wsl][i, j] = v
+                    continue
+                for j in range(NP):
                     if i < BLK and j < BLK:
                         continue
-                    g = _prev_entry(pb, pb.F[n - 1], pending, p0w, i, j)
-                    ri, cj = p0w <= i < p0w + BLK, p0w <= j < p0w + BLK
-                    if pending and ri != cj:
+                    g_ = _prev_entry(pb, pb.F[n - 1], pending, p0w, i, j)
+                    colP = p0w <= j < p0w + BLK
+                    if pending and rowP != colP:
                         assert Wsl == n - 1
-                        pb.F[Wsl][i, j] = g
-                    pb.F[n][i, j] = _slice_entry(pb.d, n, i, j, g)
-                else:
-                    half = 64 * (NP - 64)
-                    if e < half:
-                        a, jj = e // (NP - 64), e % (NP - 64)
-                        i, j = p0w + a, (jj if jj < p0w else jj + 64)
-                        _chk((a, BLK), (j, NP))
-                        v = pb.R[a, j]
-                    else:
-                        e2 = e - half
-                        ii, b = e2 >> 6, e2 & 63
-                        i, j = (ii if ii < p0w else ii + 64), p0w + b
-                        _chk((i, NP))
-                        v = pb.C[i, b]
-                    assert not np.isnan(v)
-                    pb.F[Wsl][i, j] = v
+                        pb.F[Wsl][i, j] = g_
+                    pb.F[n][i, j] = _slice_entry(pb.d, n, i, j, g_)
 
 
 def k2(probs, r):
@@ -202,7 +200,7 @@ def test_dyson_rounds_match_the_recursion(widths):
     probs.sort(key=lambda pb: -pb.T)  # the engine's order
     rounds = max(pb.T for pb in probs) + 1
     for r in range(rounds):
-        k1(probs, r, CH=5000)
+        k1(probs, r)
         k2(probs, r)
         k3(probs, r)
     for pb in probs:
@@ -252,22 +250,20 @@ def test_one_sweep_through_exact_inverses_solves_the_system():
         assert np.abs(H @ psi - np.vstack(b)).max() < 1e-10 * np.abs(psi).max()
 
 
-def test_elementwise_chunking_covers_each_entry_once():
-    # the write-back mapping of a block's rows and columns: every entry outside
-    # the block in those rows and columns exactly once, nothing else
+def test_write_back_covers_the_block_rows_and_columns_once():
+    # the row pass's write-back of block [p0w, p0w + 64): every entry outside
+    # the block in its rows and columns exactly once, nothing else
     for NP in (128, 192, 320):
         for p0w in range(0, NP, 64):
             seen = np.zeros((NP, NP), int)
-            half = 64 * (NP - 64)
-            for e in range(128 * (NP - 64)):
-                if e < half:
-                    a, jj = e // (NP - 64), e % (NP - 64)
-                    i, j = p0w + a, (jj if jj < p0w else jj + 64)
+            for i in range(NP):
+                if p0w <= i < p0w + BLK:
+                    for j in range(NP):
+                        if not p0w <= j < p0w + BLK:
+                            seen[i, j] += 1
                 else:
-                    e2 = e - half
-                    ii, b = e2 >> 6, e2 & 63
-                    i, j = (ii if ii < p0w else ii + 64), p0w + b
-                seen[i, j] += 1
+                    for lane in range(BLK):
+                        seen[i, p0w + lane] += 1
             rows = np.zeros(NP, bool)
             rows[p0w:p0w + 64] = True
             want = (rows[:, None] ^ rows[None, :]).astype(int)
