@@ -535,27 +535,32 @@ def dense_gemm(A: np.ndarray, B: np.ndarray, precision: str = "fp64", device: in
 
 
 def transmission_gradient(syst: dict, out=("R",), in_=("L",), *, three_d: bool = False, checkpoint_every: int = 32,
-                          device: int = -1, return_info: bool = False):
+                          device: int = -1, return_info: bool = False, precision: str = "fp64",
+                          options: dict | None = None):
     """T = sum |t_ij|^2 over the selected transmission block and dT/d eps.
 
     ``out``/``in_`` follow cal_smatrix_RGF (cells or structs) but must select
     inputs on one side and outputs on the other.  Returns ``(T, dT/dRe(eps),
-    dT/dIm(eps))`` with the gradients shaped like ``syst['epsilon']``.  The
-    adjoint keeps every ``checkpoint_every``-th slice inverse and recomputes
-    the rest segment by segment (0 keeps all of them).
+    dT/dIm(eps))`` with the gradients shaped like ``syst['epsilon']``.
+    ``precision`` 'fp64': the adjoint keeps every ``checkpoint_every``-th
+    slice inverse and recomputes the rest segment by segment (0 keeps all of
+    them).  'fp32_refined': FP32 slice inverses (all kept) with the forward
+    fields and the adjoint rows each refined to FP64 accuracy.
     """
     prob = _problem_3d(syst) if three_d else _problem_2d(syst)
     lib = L.load_library()
     nprop = _query(prob)
     sel = _Sel(_parse_selection(list(out) if isinstance(out, tuple) else out, "out", nprop),
                _parse_selection(list(in_) if isinstance(in_, tuple) else in_, "in", nprop))
-    shape = np.asarray(syst["epsilon"]).shape
+    shape = tuple(syst["epsilon"].shape)
     n = int(np.prod(shape))
     gre = np.zeros(n)
     gim = np.zeros(n)
     T = C.c_double()
     info = L.Info()
-    o = _options("fp64")
+    o = _options(precision, **(options or {}))
+    if prob.device is not None:
+        device = prob.device
     with L.using(device) as h:
         st = lib.rgf_transmission_gradient(h, C.byref(prob.p), C.byref(sel.s), C.byref(o), int(checkpoint_every),
                                            C.byref(T), L.dptr(gre), L.dptr(gim), C.byref(info))

@@ -692,14 +692,156 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
     }
 }
 
+// ---------------------------------------------------------------- adjoint gradient (FP32 + refinement)
+// A0 transposed (CSR): the row (adjoint) residual needs A0's columns.
+HostCSR transpose_csr(const HostCSR& a) {
+    HostCSR t;
+    t.n = a.n;
+    t.rowptr.assign((size_t)a.n + 1, 0);
+    for (int32_t q : a.col) ++t.rowptr[(size_t)q + 1];
+    for (int32_t i = 0; i < a.n; ++i) t.rowptr[(size_t)i + 1] += t.rowptr[(size_t)i];
+    t.col.resize(a.col.size());
+    t.vre.resize(a.col.size());
+    t.vim.resize(a.col.size());
+    std::vector<int32_t> at(t.rowptr.begin(), t.rowptr.end() - 1);
+    for (int32_t i = 0; i < a.n; ++i)
+        for (int32_t q = a.rowptr[(size_t)i]; q < a.rowptr[(size_t)i + 1]; ++q) {
+            const int32_t k = a.col[(size_t)q], dst = at[(size_t)k]++;
+            t.col[(size_t)dst] = i;
+            t.vre[(size_t)dst] = a.vre[(size_t)q];
+            t.vim[(size_t)dst] = a.vim[(size_t)q];
+        }
+    return t;
+}
+
+// T = sum |t_ij|^2 and dT/d eps with the mixed path (sweep-side names as in
+// run_gradient).  The forward fields psi = H^-1 b (b = B_L at slice 0) are
+// run_refined's refined solution (named buffer "psi", S x N x m).  The
+// adjoint rows y = c H^-1, c = t^H C_R at slice L+1 (so that y_n =
+// t^H C_R G_{L+1,n}), are refined the same way in row form through the same
+// stored FP32 inverses: forward z_n = (r_n + z_{n-1}) g_n, backward
+// d_n = z_n + d_{n+1} g_n, with the FP64 residual r = c - y H (the stencil
+// transposed, the lead terms y_0 G0L and y_{L+1} G0R).  The gradient is the
+// contraction of run_gradient: u_i = k^2 sum_j psi_n[i,j] y_n[j,i].
+void run_gradient_refined(rgf_context* c, Work& w, const rgf_options& o, double& T, std::vector<double>& gre,
+                          std::vector<double>& gim) {
+    run_refined(c, w, o);
+    const int64_t N = w.N, L = w.L, S = L + 2;
+    const int64_t mL = (int64_t)w.inL.size(), m = (mL + 3) / 4 * 4, no = (int64_t)w.outR.size();
+    hipStream_t s = c->stream;
+    const double t0 = now();
+    T = 0;
+    std::vector<cd> tH((size_t)m * no, cd(0, 0));  // rows >= mL stay zero
+    for (int64_t i = 0; i < no; ++i)
+        for (int64_t j = 0; j < mL; ++j) {
+            const cd t = cd(0, -2) * w.TRL[(size_t)(i * mL + j)];
+            T += std::norm(t);
+            tH[(size_t)(j * no + i)] = std::conj(t);
+        }
+    CMat<double> TH = dmat<double>(c, "TH", m, no), Cadj = dmat<double>(c, "adj_c", m, N);
+    upload(c, tH, m, no, TH);
+    cgemm<double>(s, m, N, no, TH, w.CR, Cadj, cd(1, 0), 0);
+    DevCSR AT;
+    {
+        const HostCSR h = transpose_csr(slice_stencil(*w.ch));
+        AT.n = h.n;
+        AT.nnz = (int32_t)h.col.size();
+        AT.rowptr = dbuf<int32_t>(c, "AT_rowptr", h.rowptr.size());
+        AT.col = dbuf<int32_t>(c, "AT_col", h.col.size());
+        AT.vre = dbuf<double>(c, "AT_vre", h.vre.size());
+        AT.vim = dbuf<double>(c, "AT_vim", h.vim.size());
+        RGF_HIP(hipMemcpyAsync(AT.rowptr, h.rowptr.data(), h.rowptr.size() * 4, hipMemcpyHostToDevice, s));
+        RGF_HIP(hipMemcpyAsync(AT.col, h.col.data(), h.col.size() * 4, hipMemcpyHostToDevice, s));
+        RGF_HIP(hipMemcpyAsync(AT.vre, h.vre.data(), h.vre.size() * 8, hipMemcpyHostToDevice, s));
+        RGF_HIP(hipMemcpyAsync(AT.vim, h.vim.data(), h.vim.size() * 8, hipMemcpyHostToDevice, s));
+        RGF_HIP(hipStreamSynchronize(s));
+    }
+    const size_t SE = (size_t)N * m;
+    float* F = (float*)c->bufs["factors"].p;
+    auto Fn = [&](int64_t n) {
+        return CMat<float>(F + (size_t)n * N * N * 2, F + (size_t)n * N * N * 2 + (size_t)N * N, N);
+    };
+    double* y = dbuf<double>(c, "adj_y", 2 * S * SE);
+    double* r = dbuf<double>(c, "adj_r", 2 * S * SE);
+    float* d = dbuf<float>(c, "adj_d", 2 * S * SE);
+    CMat<float> tt = dmat<float>(c, "adj_t", m, N);
+    auto Y = [&](double* base, int64_t n) { return CMat<double>(base + n * SE, base + S * SE + n * SE, N); };
+    auto D = [&](int64_t n) { return CMat<float>(d + n * SE, d + S * SE + n * SE, N); };
+    auto wk = splitk_ws<float>(c);
+    auto wk64 = splitk_ws<double>(c);
+    const int sweep3m = (o.complex_3m >> 1) & 1;
+    const double cmax0 = max_abs(s, (int64_t)SE, Cadj.re, Cadj.im, c->d_scratch);
+    const double cmax = cmax0 == 0 ? 1.0 : cmax0;
+    const int maxit = o.max_refine_iter > 0 ? o.max_refine_iter : 6;
+    const double tol = o.refine_tol > 0 ? o.refine_tol : 1e-13;
+    double rel = 1.0;
+    int it = 0;
+    for (;; ++it) {
+        double rmax;
+        if (it == 0) {
+            rmax = cmax0;  // y = 0: r = c, at slice L+1 only
+        } else {
+            residual_rows(s, N, L, m, y, y + S * SE, r, r + S * SE, w.eps_re, w.eps_im, w.epsL, w.epsR, w.k2, AT,
+                          Cadj.re, Cadj.im);
+            cgemm<double>(s, m, N, N, Y(y, 0), w.G0L, Y(r, 0), cd(1, 0), 1, 0, 0, 0, wk64.first, wk64.second);
+            cgemm<double>(s, m, N, N, Y(y, L + 1), w.G0R, Y(r, L + 1), cd(1, 0), 1, 0, 0, 0, wk64.first,
+                          wk64.second);
+            rmax = max_abs(s, S * (int64_t)SE, r, r + S * SE, c->d_scratch);
+        }
+        rel = rmax / cmax;
+        if (!std::isfinite(rel)) throw Error(RGF_ERR_NUMERICAL, "non-finite residual in the adjoint refinement");
+        if (rel <= tol || it >= maxit || rmax == 0) break;
+        const double sc = 1.0 / rmax;
+        if (it == 0) {
+            // r vanishes before slice L+1: z_n = 0 there, z_{L+1} = (c / rmax) g_{L+1}
+            RGF_HIP(hipMemsetAsync(d, 0, 2 * S * SE * sizeof(float), s));
+            convert<float, double>(s, m, N, Cadj, tt, sc);
+            cgemm<float>(s, m, N, N, tt, Fn(S - 1), D(S - 1), cd(1, 0), 0, 0, 0, sweep3m, wk.first, wk.second);
+        } else {
+            convert<float, double>(s, m, N, Y(r, 0), tt, sc);
+            for (int64_t n = 0; n < S; ++n) {
+                cgemm<float>(s, m, N, N, tt, Fn(n), D(n), cd(1, 0), 0, 0, 0, sweep3m, wk.first, wk.second);
+                if (n + 1 < S) {
+                    convert<float, double>(s, m, N, Y(r, n + 1), tt, sc);
+                    add_mat<float>(s, m, N, tt, D(n), tt);
+                }
+            }
+        }
+        for (int64_t n = S - 2; n >= 0; --n)
+            cgemm<float>(s, m, N, N, D(n + 1), Fn(n), D(n), cd(1, 0), 1, 0, 0, sweep3m, wk.first, wk.second);
+        if (it == 0) RGF_HIP(hipMemsetAsync(y, 0, 2 * S * SE * sizeof(double), s));
+        accumulate<float>(s, S * m, N, CMat<float>(d, d + S * SE, N), CMat<double>(y, y + S * SE, N), rmax);
+    }
+    if (rel > tol && it >= maxit) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "adjoint refinement stopped at relative residual %.3e (tolerance %.1e)", rel, tol);
+        throw Error(RGF_ERR_NOT_CONVERGED, buf);
+    }
+    // gradient: psi_n (N x m, ld m) and y_n (m x N, ld N), slices 1..L
+    double* psi = (double*)c->bufs["psi"].p;
+    double* dg = dbuf<double>(c, "grad", 2 * (size_t)N * std::max<int64_t>(L, 1));
+    for (int64_t n = 1; n <= L; ++n)
+        grad_contract(s, N, m, CMat<double>(psi + n * SE, psi + S * SE + n * SE, m), Y(y, n), w.k2,
+                      dg + (n - 1) * N, dg + (size_t)N * L + (n - 1) * N);
+    gre.assign((size_t)N * L, 0);
+    gim.assign((size_t)N * L, 0);
+    if (L > 0) {
+        RGF_HIP(hipMemcpyAsync(gre.data(), dg, sizeof(double) * N * L, hipMemcpyDeviceToHost, s));
+        RGF_HIP(hipMemcpyAsync(gim.data(), dg + (size_t)N * L, sizeof(double) * N * L, hipMemcpyDeviceToHost, s));
+    }
+    RGF_HIP(hipStreamSynchronize(s));
+    w.t_refine += now() - t0;
+    w.refine_iters += it;  // forward and adjoint correction sweeps
+}
+
 void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt, int every,
                   double* T_out, double* gre_out, double* gim_out, rgf_info* info) {
     const double t0 = now();
     rgf_options o;
     rgf_default_options(&o);
     if (opt) o = *opt;
-    if (o.precision != RGF_PRECISION_FP64)
-        throw Error(RGF_ERR_INVALID_ARGUMENT, "the transmission gradient is computed in FP64");
+    if (o.precision != RGF_PRECISION_FP64 && o.precision != RGF_PRECISION_FP32_REFINED)
+        throw Error(RGF_ERR_INVALID_ARGUMENT, "unknown precision");
     Channels ch = setup_channels(p);
     Selection sel = make_selection(s, ch);
     fill_info(info, ch, sel);
@@ -719,7 +861,10 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
     w.t_setup = now() - t0;
     double T = 0;
     std::vector<double> gre, gim;
-    run_gradient(c, w, every, T, gre, gim);
+    if (o.precision == RGF_PRECISION_FP64)
+        run_gradient(c, w, every, T, gre, gim);
+    else
+        run_gradient_refined(c, w, o, T, gre, gim);
     if (T_out) *T_out = T;
     const int64_t N = w.N, L = w.L;
     for (int64_t n = 0; n < L; ++n) {
@@ -733,6 +878,9 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
         info->swept_right_to_left = w.mirrored;
         info->seconds_setup = w.t_setup;
         info->seconds_slices = w.t_slices;
+        info->seconds_refine = w.t_refine;
+        info->refine_iters = w.refine_iters;
+        info->refine_residual = w.refine_residual;
         info->flops_slices = w.flops;
         info->max_multiplier = w.max_multiplier;
         if (timer) {

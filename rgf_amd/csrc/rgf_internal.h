@@ -168,6 +168,14 @@ void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const doub
                       const double* b0_re, const double* b0_im, int64_t mL, const double* b1_re,
                       const double* b1_im, int64_t mR);
 
+// Residual of the row (adjoint) system y H = c over slices 0..L+1, y_n and
+// r_n m x N (FP64, slice stride N m): r_n = c_n - y_n D_n + y_{n-1} + y_{n+1},
+// c non-zero at slice L+1 only (m x N); AT = A0 transposed.  The lead terms
+// y_0 G0L and y_{L+1} G0R are added by the caller.
+void residual_rows(hipStream_t s, int64_t N, int64_t L, int64_t m, const double* y_re, const double* y_im,
+                   double* r_re, double* r_im, const double* eps_re, const double* eps_im, cd epsL, cd epsR, cd k2,
+                   const DevCSR& AT, const double* c_re, const double* c_im);
+
 // Adjoint contraction for one slice: with u_i = k2 * sum_j psi[i][j] phi[j][i]
 // (psi N x m, phi m x N), dT/dRe(eps_i) = 4 Im(u_i) and dT/dIm(eps_i) = 4 Re(u_i).
 void grad_contract(hipStream_t s, int64_t N, int64_t m, CMat<double> psi, CMat<double> phi, cd k2, double* gre,

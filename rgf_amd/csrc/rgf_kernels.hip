@@ -2470,6 +2470,73 @@ void residual_stencil(hipStream_t s, int64_t N, int64_t L, int64_t m, const doub
     RGF_HIP(hipGetLastError());
 }
 
+// Residual of the row system y H = c (the adjoint of H psi = b): y_n is
+// m x N, r_n = c_n - y_n D_n + y_{n-1} + y_{n+1} with D_n = A0 - k2 diag(eps_n)
+// (slices 0 and L+1: eps_L, eps_R; the caller adds the lead terms y_0 G0L
+// and y_{L+1} G0R).  y_n D_n needs the columns of A0: AT is A0 transposed
+// (CSR).  c is non-zero at slice L+1 only (m x N).
+__global__ void residual_rows_kernel(int64_t N, int64_t L, int64_t m, const double* yr, const double* yim,
+                                     double* rr, double* ri, const double* er, const double* ei, double eLr,
+                                     double eLi, double eRr, double eRi, double k2r, double k2i,
+                                     const int32_t* rowptr, const int32_t* col, const double* vre,
+                                     const double* vim, const double* cr, const double* ci, int64_t n0) {
+    const int64_t S = N * m;
+    const int64_t n = n0 + blockIdx.y;
+    const unsigned uN = (unsigned)N;
+    for (unsigned rem = blockIdx.x * blockDim.x + threadIdx.x; rem < (unsigned)S; rem += gridDim.x * blockDim.x) {
+        const unsigned j = rem / uN, i = rem - j * uN;
+        const int64_t e = n * S + rem;
+        double xr, xi;
+        if (n == 0) {
+            xr = eLr;
+            xi = eLi;
+        } else if (n == L + 1) {
+            xr = eRr;
+            xi = eRi;
+        } else {
+            xr = er[(n - 1) * N + i];
+            xi = ei ? ei[(n - 1) * N + i] : 0.0;
+        }
+        const double ker = k2r * xr - k2i * xi, kei = k2r * xi + k2i * xr;
+        const double yr0 = yr[e], yi0 = yim[e];
+        double accr = ker * yr0 - kei * yi0;
+        double acci = ker * yi0 + kei * yr0;
+        for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+            const int64_t o = n * S + (int64_t)j * N + col[q];
+            const double ar = vre[q], ai = vim[q];
+            accr -= ar * yr[o] - ai * yim[o];
+            acci -= ar * yim[o] + ai * yr[o];
+        }
+        if (n > 0) {
+            accr += yr[e - S];
+            acci += yim[e - S];
+        }
+        if (n < L + 1) {
+            accr += yr[e + S];
+            acci += yim[e + S];
+        }
+        if (n == L + 1) {
+            accr += cr[rem];
+            acci += ci[rem];
+        }
+        rr[e] = accr;
+        ri[e] = acci;
+    }
+}
+
+void residual_rows(hipStream_t s, int64_t N, int64_t L, int64_t m, const double* y_re, const double* y_im,
+                   double* r_re, double* r_im, const double* eps_re, const double* eps_im, cd epsL, cd epsR, cd k2,
+                   const DevCSR& AT, const double* c_re, const double* c_im) {
+    if ((uint64_t)N * (uint64_t)m >= (1ull << 31)) throw Error(RGF_ERR_INVALID_ARGUMENT, "slice too large");
+    for (int64_t n0 = 0; n0 < L + 2; n0 += 65535) {
+        const dim3 grid((unsigned)grid_for(N * m, 256, 1024), (unsigned)std::min<int64_t>(65535, L + 2 - n0));
+        hipLaunchKernelGGL(residual_rows_kernel, grid, dim3(256), 0, s, N, L, m, y_re, y_im, r_re, r_im, eps_re,
+                           eps_im, epsL.real(), epsL.imag(), epsR.real(), epsR.imag(), k2.real(), k2.imag(),
+                           AT.rowptr, AT.col, AT.vre, AT.vim, c_re, c_im, n0);
+    }
+    RGF_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ field profiles
 // out[(i + N * np) + N * (L + 2) * a] (interleaved complex) = psi at physical
 // slice np, transverse site i, input a: internal slice (mirrored ? L+1-np : np),
