@@ -62,6 +62,10 @@ struct HostProb {
     bool ready = false, failed = false, converged = false;
     int iters = 0;
     double bmax = 0, rel = 1;
+    // refine_s_tol: the S blocks of the last sweep and the error estimate
+    std::vector<double> Tprev;
+    double s_err = 0;
+    bool s_done = false;
     int64_t T() const { return (L + 1) * nb; }
     int64_t S() const { return L + 2; }
 };
@@ -372,6 +376,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
     const double fl3 = sweep3m ? 6.0 : 8.0;
     const int maxit = o.max_refine_iter > 0 ? o.max_refine_iter : 6;
     const double tol = o.refine_tol > 0 ? o.refine_tol : 1e-13;
+    const bool s_stop = o.refine_s_tol > 0;
     std::vector<unsigned long long> rm(np);
     for (int it = 0;; ++it) {
         std::vector<size_t> act;
@@ -393,6 +398,42 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
         batch_resmax(s, dtab, imaps + nlead, nres, first);
         RGF_HIP(hipMemcpyAsync(rm.data(), rmax, 8 * np, hipMemcpyDeviceToHost, s));
         RGF_HIP(hipStreamSynchronize(s));
+        if (s_stop && it >= 1) {
+            // the S-error estimate of run_refined: with linear convergence the
+            // latest S is within q / (1 - q) of the change over the last sweep
+            // (q the ratio of the last two residuals), doubled; S = -2i T
+            Maps pm;
+            const int npj = pm.append(act, [&](size_t k) {
+                const HostProb& hp = *ps[k];
+                return (ceil_div(hp.npo, 64) + ceil_div(hp.nqo, 64)) * ceil_div(hp.m, 64);
+            });
+            int32_t* pmaps = dbuf<int32_t>(h, "batch_pmaps", pm.v.size() + 64);
+            RGF_HIP(hipMemcpyAsync(pmaps, pm.v.data(), 4 * pm.v.size(), hipMemcpyHostToDevice, s));
+            batch_project(s, dtab, pmaps, npj);
+            RGF_HIP(hipStreamSynchronize(s));
+            for (size_t k : act) {
+                HostProb& hp = *ps[k];
+                std::vector<double> Tc((size_t)2 * (hp.npo + hp.nqo) * hp.m);
+                if (!Tc.empty())
+                    RGF_HIP(hipMemcpy(Tc.data(), arena + hp.base + hp.o_T, sizeof(double) * Tc.size(),
+                                      hipMemcpyDeviceToHost));
+                double r;
+                memcpy(&r, &rm[k], 8);
+                const double rel = r / hp.bmax;
+                if (it >= 2 && Tc.size() == hp.Tprev.size()) {
+                    double dT = 0;
+                    const size_t h2 = Tc.size() / 2;
+                    for (size_t e = 0; e < h2; ++e)
+                        dT = std::max(dT, std::hypot(Tc[e] - hp.Tprev[e], Tc[h2 + e] - hp.Tprev[h2 + e]));
+                    const double q = rel / hp.rel;  // hp.rel: the previous sweep's residual
+                    if (q < 0.5) {
+                        hp.s_err = 2.0 * (2.0 * dT) * q / (1.0 - q);
+                        hp.s_done = hp.s_err <= o.refine_s_tol;
+                    }
+                }
+                hp.Tprev = std::move(Tc);
+            }
+        }
         std::vector<size_t> go;
         for (size_t k : act) {
             HostProb& hp = *ps[k];
@@ -405,7 +446,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
                 hp.failed = true;
                 sts[(size_t)hp.user] = RGF_ERR_NUMERICAL;
                 msgs[(size_t)hp.user] = "non-finite residual in refinement";
-            } else if (hp.rel <= tol || r == 0) {
+            } else if (hp.rel <= tol || r == 0 || hp.s_done) {
                 hp.converged = true;
             } else if (it >= maxit) {
                 hp.failed = true;
@@ -505,6 +546,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
             rgf_info& in = info[hp.user];
             in.refine_iters = hp.iters;
             in.refine_residual = hp.rel;
+            in.refine_s_err_estimate = hp.s_err;
             in.seconds_setup = t_setup;
             in.seconds_slices = t_slices;
             in.seconds_refine = t_refine;

@@ -136,3 +136,18 @@ def test_batched_sweep_full_size():
         perm = np.concatenate([ch["L"]["ind_prop_conj"], ch["L"]["N_prop"] + ch["R"]["ind_prop_conj"]]) - 1
         St = S[perm]
         assert np.abs(St.T - St).max() < 1e-9
+
+
+def test_engine_s_error_stop():
+    """refine_s_tol through the engine: the estimate bounds the error against
+    the FP64 path and stays under the tolerance, with fewer sweeps than the
+    residual criterion needs."""
+    items = _ragged_items(9)[:6]
+    S64 = [_single(s, o_, i_)[0] for s, o_, i_ in items]
+    S32, infos = rgf_amd.cal_smatrix_RGF_batched(items, precision="fp32_refined", return_info=True,
+                                                 options={"refine_s_tol": 1e-9, "refine_tol": 1e-16})
+    for a, b, info in zip(S32, S64, infos):
+        err = float(np.abs(a - b).max())
+        assert err <= 1e-9
+        assert info["refine_s_err_estimate"] <= 1e-9
+        assert info["refine_iters"] <= 3
