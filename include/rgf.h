@@ -243,6 +243,29 @@ rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem* p,
  * rgf_dense_gemm: C = A B, (m x k)(k x n), interleaved column-major. */
 #define RGF_DENSE_3M 0x100
 rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double* A, int32_t block_size);
+
+/* No counterpart in the reference.  rgf_dense_inverse for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops): the same
+ * slice-inverse kernels on a matrix already in device memory, with their
+ * workspace in a static device array of RGF_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends); the call allocates nothing.
+ *   A_re, A_im: the n x n matrix, planar (real and imaginary planes), row-major
+ *     with row stride lda >= n elements, inverted in place; elements are
+ *     double for RGF_PRECISION_FP64, float for RGF_PRECISION_FP32_REFINED
+ *     (| RGF_DENSE_3M as in rgf_dense_inverse).
+ *   status: device memory for two int32 words, or NULL.  Given, the call is
+ *     asynchronous on `stream` and leaves [0] != 0 on a zero or non-finite
+ *     pivot and in [1] the largest multiplier squared (float bits, see
+ *     rgf_info.max_multiplier); NULL: the call waits for the inverse and
+ *     returns RGF_ERR_NUMERICAL on a zero pivot (or, in FP64, a multiplier
+ *     above 1e6), as rgf_dense_inverse does.
+ *   stream: a HIP stream (hipStream_t), or NULL for the handle's stream.
+ * n beyond the workspace (about 8160 at the default block sizes) is refused
+ * with RGF_ERR_INVALID_ARGUMENT.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on a GPU (from any handle) race. */
+#define RGF_NOALLOC_SCRATCH_BYTES (32u << 20)
+rgf_status rgf_dense_inverse_noalloc(rgf_handle h, int32_t precision, int64_t n, void* A_re, void* A_im, int64_t lda,
+                                     int32_t block_size, int32_t* status, void* stream);
 rgf_status rgf_dense_gemm(rgf_handle h, int32_t precision, int64_t m, int64_t n, int64_t k,
                           const double* A, const double* B, double* C);
 

@@ -104,6 +104,8 @@ SIGNATURES = {
     "rgf_transmission_gradient": (C.c_int, [C.c_void_p, C.POINTER(Problem), C.POINTER(Selection),
                                             C.POINTER(Options), C.c_int32, _dp, _dp, _dp, C.POINTER(Info)]),
     "rgf_dense_inverse": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, _dp, C.c_int32]),
+    "rgf_dense_inverse_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                            C.c_int32, C.c_void_p, C.c_void_p]),
     "rgf_dense_gemm": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _dp, _dp, _dp]),
 }
 

@@ -1092,6 +1092,7 @@ rgf_status rgf_create(rgf_handle* h, int32_t device) {
         for (auto& e : c->ov.ev) RGF_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
         RGF_HIP(hipMalloc(&c->d_status, 256));
         RGF_HIP(hipMalloc(&c->d_scratch, 256));
+        c->scratch = noalloc_scratch(device);  // once per device, with it current
         *h = c.release();
     });
 }
@@ -1322,6 +1323,58 @@ rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double*
         }
         check_status(h, true, precision == RGF_PRECISION_FP64);
         download_colmajor(h, M, n, n, A);
+    });
+}
+
+rgf_status rgf_dense_inverse_noalloc(rgf_handle h, int32_t precision, int64_t n, void* A_re, void* A_im, int64_t lda,
+                                     int32_t block_size, int32_t* status, void* stream) {
+    return guard(h, [&] {
+        if (!h || !A_re || !A_im || n <= 0 || lda < n)
+            throw Error(RGF_ERR_INVALID_ARGUMENT, "bad no-allocation dense inverse arguments");
+        const int use3m = (precision & RGF_DENSE_3M) ? 1 : 0;
+        precision &= ~RGF_DENSE_3M;
+        if (precision != RGF_PRECISION_FP64 && precision != RGF_PRECISION_FP32_REFINED)
+            throw Error(RGF_ERR_INVALID_ARGUMENT, "unknown precision");
+        if (use3m && precision == RGF_PRECISION_FP64)
+            throw Error(RGF_ERR_INVALID_ARGUMENT, "RGF_DENSE_3M applies to the FP32 kernels");
+        const bool fp64 = precision == RGF_PRECISION_FP64;
+        const int b = block_size > 0 ? block_size : default_block(precision);
+        if (b > (fp64 ? diag_block_max<double>() : diag_block_max<float>()) || b % 16 != 0)
+            throw Error(RGF_ERR_INVALID_ARGUMENT, "invalid inverse block size");
+        const size_t need = 256 + (fp64 ? sizeof(double) * inverse_workspace_elems<double>(n, b)
+                                        : sizeof(float) * inverse_workspace_elems<float>(n, b));
+        if (need > RGF_NOALLOC_SCRATCH_BYTES) {
+            char buf[200];
+            snprintf(buf, sizeof buf,
+                     "n = %lld needs %zu bytes of workspace; the no-allocation entry point has %u "
+                     "(use rgf_dense_inverse)",
+                     (long long)n, need, (unsigned)RGF_NOALLOC_SCRATCH_BYTES);
+            throw Error(RGF_ERR_INVALID_ARGUMENT, buf);
+        }
+        if (!h->scratch) throw Error(RGF_ERR_DEVICE, "no-allocation workspace unavailable on this device");
+        RGF_HIP(hipSetDevice(h->device));
+        hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+        int* st = status ? status : (int*)h->scratch;
+        zero_words(s, st, 2);
+        void* ws = (char*)h->scratch + 256;
+        if (fp64)
+            invert_inplace<double>(s, n, CMat<double>((double*)A_re, (double*)A_im, lda), b, (double*)ws, st, 0,
+                                   nullptr, &h->ov);
+        else
+            invert_inplace<float>(s, n, CMat<float>((float*)A_re, (float*)A_im, lda), b, (float*)ws, st, use3m,
+                                  nullptr, &h->ov);
+        if (!status) {
+            int w[2] = {0, 0};
+            RGF_HIP(hipMemcpyAsync(w, st, sizeof w, hipMemcpyDeviceToHost, s));
+            RGF_HIP(hipStreamSynchronize(s));
+            if (w[0]) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+            float m2;
+            memcpy(&m2, &w[1], sizeof m2);
+            if (fp64 && !(std::sqrt((double)m2) <= multiplier_limit(true)))
+                throw Error(RGF_ERR_NUMERICAL,
+                            "ill-conditioned diagonal block in a slice inverse (pivoting is confined to the "
+                            "diagonal block, so the inverse would be inaccurate)");
+        }
     });
 }
 

@@ -26,6 +26,7 @@ struct rgf_context {
     double* d_scratch = nullptr;
     std::vector<rgf_context*> workers;  // worker contexts of rgf_solve_batched
     std::vector<hipEvent_t> timer_events;  // kept across solves (options.profile)
+    void* scratch = nullptr;               // the device's no-allocation workspace
 };
 
 namespace rgf {

@@ -181,6 +181,17 @@ void residual_rows(hipStream_t s, int64_t N, int64_t L, int64_t m, const double*
 void grad_contract(hipStream_t s, int64_t N, int64_t m, CMat<double> psi, CMat<double> phi, cd k2, double* gre,
                    double* gim);
 
+// Address of the no-allocation workspace (RGF_NOALLOC_SCRATCH_BYTES) on a
+// device, looked up once per device and kept.
+void* noalloc_scratch(int device);
+// p[0..n) = 0 with a kernel (n <= 64; capturable, unlike a memset node)
+void zero_words(hipStream_t s, int* p, int n);
+// slice-inverse block size per precision (FP64 64, FP32 128)
+template <typename T>
+constexpr int diag_block_max() {
+    return sizeof(T) == 8 ? 64 : 128;
+}
+
 // max |x| over a planar FP64 array (result to host, synchronizes the stream).
 double max_abs(hipStream_t s, int64_t count, const double* re, const double* im, double* dev_scratch);
 

@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 namespace rgf {
@@ -1867,7 +1869,7 @@ __global__ void panel_copy_kernel(const T* Are, const T* Aim, int64_t lda, int64
 
 template <typename T>
 static constexpr int diag_bmax() {
-    return sizeof(T) == 8 ? 64 : 128;  // the diagonal block lives in LDS
+    return diag_block_max<T>();  // the diagonal block lives in LDS
 }
 
 void KernelTimer::begin(hipStream_t s) {
@@ -2616,6 +2618,38 @@ double max_abs(hipStream_t s, int64_t count, const double* re, const double* im,
     RGF_HIP(hipMemcpyAsync(&h, dev_scratch, sizeof(double), hipMemcpyDeviceToHost, s));
     RGF_HIP(hipStreamSynchronize(s));
     return h;
+}
+
+// ------------------------------------------------------------------ no-allocation workspace
+// The device memory of rgf_dense_inverse_noalloc: 256 bytes of status words,
+// then the slice-inverse workspace.  Each GPU holds its own copy at its own
+// address (noalloc_scratch) until the process ends.
+static __device__ __align__(256) unsigned char scratch[RGF_NOALLOC_SCRATCH_BYTES];
+
+void* noalloc_scratch(int device) {
+    static std::mutex mu;
+    static std::map<int, void*> ptrs;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = ptrs.find(device);
+    if (it != ptrs.end()) return it->second;
+    int cur = 0;
+    RGF_HIP(hipGetDevice(&cur));
+    if (cur != device) RGF_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    RGF_HIP(e);
+    ptrs[device] = p;
+    return p;
+}
+
+__global__ void zero_words_kernel(int* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+void zero_words(hipStream_t s, int* p, int n) {
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, p, n);
+    RGF_HIP(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ instantiations
