@@ -188,6 +188,16 @@ rgf_status rgf_query(const rgf_problem* p, const rgf_selection* sel,
 /* Host-only: lead channels; replaces setup_channels.m / setup_channels_3D.m. */
 rgf_status rgf_setup_channels(const rgf_problem* p, rgf_channels* out);
 
+/* Host-only: transverse mode profiles at given wavenumbers, what
+ * channels.fun_phi returns (setup_channels.m:121-156 for 2D,
+ * setup_channels_3D.m:101-123 for 3D).  Reads p's ndim, transverse dims, bc
+ * and bloch_ka (and checks p as rgf_setup_channels does).  phi_out: complex,
+ * interleaved; 2D: dims[0] x count column-major, column c the profile at
+ * k1[c]; 3D: (dims[0] * dims[1]) x count, column c the dims[0] x dims[1]
+ * profile (x fast) at (k1[c], k2[c]). */
+rgf_status rgf_transverse_profiles(const rgf_problem* p, int64_t count, const double* k1, const double* k2,
+                                   double* phi_out);
+
 /* Host-only: 1D Laplacian of build_laplacian_1d.m as a dense n x n complex
  * matrix (interleaved, column-major, 2 n^2 doubles). */
 rgf_status rgf_build_laplacian_1d(int64_t n, int32_t bc, double ka, double* A_out);

@@ -264,11 +264,11 @@ def setup_channels_3D(nx: int, ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> 
     if kind == "Bloch":
         zx = (nx + 1) // 2 if nx % 2 == 1 else nx // 2
         zy = (ny + 1) // 2 if ny % 2 == 1 else ny // 2
-        kx1 = kxa / nx + (ix - zx) * (2 * math.pi / nx)                   # (:421)
-        ky1 = kya / ny + (iy - zy) * (2 * math.pi / ny)                   # (:422)
+        kx1 = kxa / nx + (ix - zx) * (2 * math.pi / nx)                   # (:115)
+        ky1 = kya / ny + (iy - zy) * (2 * math.pi / ny)                   # (:116)
     else:
-        kx1 = ix * (math.pi / (nx + 1))                                    # (:426)
-        ky1 = iy * (math.pi / (ny + 1))                                    # (:427)
+        kx1 = ix * (math.pi / (nx + 1))                                    # (:120)
+        ky1 = iy * (math.pi / (ny + 1))                                    # (:121)
     kxdx = np.repeat(kx1[:, None], ny, axis=1)
     kydx = np.repeat(ky1[None, :], nx, axis=0)
     ch = {"kxdx": kxdx, "kydx": kydx, "bc": kind, "kxa": kxa, "kya": kya, "nx": nx, "ny": ny}

@@ -94,6 +94,8 @@ SIGNATURES = {
     "rgf_trim": (C.c_int, [C.c_void_p]),
     "rgf_query": (C.c_int, [C.POINTER(Problem), C.POINTER(Selection), _ip, _ip, _ip, _ip]),
     "rgf_setup_channels": (C.c_int, [C.POINTER(Problem), C.POINTER(Channels)]),
+    "rgf_transverse_profiles": (C.c_int, [C.POINTER(Problem), C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]),
     "rgf_build_laplacian_1d": (C.c_int, [C.c_int64, C.c_int32, C.c_double, _dp]),
     "rgf_solve": (C.c_int, [C.c_void_p, C.POINTER(Problem), C.POINTER(Selection), C.POINTER(Options), _dp,
                             C.POINTER(Info)]),

@@ -15,7 +15,6 @@ All numerical work runs in librgf on the GPU; this module only marshals.
 from __future__ import annotations
 
 import ctypes as C
-import math
 import warnings
 
 import numpy as np
@@ -423,6 +422,34 @@ def _side(c, arrs, iarrs, s, conj: bool):
     return d
 
 
+def _profile_fn(prob: _Prob, N: int, ndim: int, bloch: bool, shape=None):
+    """channels.fun_phi through rgf_transverse_profiles (setup_channels.m:133-156,
+    setup_channels_3D.m:118,123): 2D fun_phi(kydx) -> N x len(kydx); 3D
+    fun_phi(kxdx, kydx) -> nx x ny for scalars (nx x ny x K for K pairs)."""
+    lib = L.load_library()
+
+    def run(k1, k2=None):
+        k1 = np.ascontiguousarray(np.atleast_1d(k1), dtype=np.float64).ravel()
+        k2 = None if k2 is None else np.ascontiguousarray(np.atleast_1d(k2), dtype=np.float64).ravel()
+        if k2 is not None and k2.size != k1.size:
+            raise ValueError("kxdx and kydx must have the same number of elements")
+        out = np.zeros(2 * N * max(k1.size, 1))
+        L.check(lib.rgf_transverse_profiles(C.byref(prob.p), k1.size, L.dptr(k1),
+                                            L.dptr(k2) if k2 is not None else None, L.dptr(out)))
+        z = out[0::2] + 1j * out[1::2]
+        z = z[:N * k1.size].reshape((N, k1.size), order="F")
+        return z if bloch else z.real
+
+    if ndim == 2:
+        return lambda kydx: run(kydx)
+
+    def fun_phi(kxdx, kydx):
+        z = run(kxdx, kydx)
+        K = z.shape[1]
+        return z.reshape(shape, order="F") if K == 1 else z.reshape((*shape, K), order="F")
+    return fun_phi
+
+
 def setup_channels(ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
     """channels = setup_channels(ny, BC, k0dx, epsilon_L, epsilon_R) (setup_channels.m:1-176).
 
@@ -435,23 +462,7 @@ def setup_channels(ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> dict:
     eR = _scalar(epsilon_R, "epsilon_R") if two_sided else _scalar(epsilon_L, "epsilon_L")
     prob = _Prob(2, (ny, 0, 1), None, _scalar(epsilon_L, "epsilon_L"), eR, _scalar(k0dx, "k0dx"), [bc], [ka])
     c, arrs, iarrs = _channels_raw(prob, ny)
-    m = np.arange(1, ny + 1)[:, None]
-    if bc == L.RGF_BC_BLOCH:
-        def fun_phi(k):
-            return np.exp(1j * m * np.atleast_1d(k)[None, :]) / math.sqrt(ny)
-    elif bc == L.RGF_BC_DIRICHLET:
-        def fun_phi(k):
-            return np.sin(m * np.atleast_1d(k)[None, :]) * math.sqrt(2 / (ny + 1))
-    elif bc == L.RGF_BC_NEUMANN:
-        def fun_phi(k):
-            k = np.atleast_1d(k)[None, :]
-            return (np.cos((m - 0.5) * k) - (k == 0) * (1 - math.sqrt(0.5))) * math.sqrt(2 / ny)
-    elif bc == L.RGF_BC_DIRICHLET_NEUMANN:
-        def fun_phi(k):
-            return np.sin(m * np.atleast_1d(k)[None, :]) * math.sqrt(2 / (ny + 0.5))
-    else:
-        def fun_phi(k):
-            return np.cos((m - 0.5) * np.atleast_1d(k)[None, :]) * math.sqrt(2 / (ny + 0.5))
+    fun_phi = _profile_fn(prob, ny, 2, bc == L.RGF_BC_BLOCH)
     ch = {"kydx": arrs["kt1"].copy(), "fun_phi": fun_phi, "L": _side(c, arrs, iarrs, "L", True)}
     if two_sided:
         ch["R"] = ch["L"] if complex(epsilon_R) == complex(epsilon_L) else _side(c, arrs, iarrs, "R", True)
@@ -471,14 +482,7 @@ def setup_channels_3D(nx: int, ny: int, BC, k0dx, epsilon_L, epsilon_R=None) -> 
     prob = _Prob(3, (nx, ny, 0), None, _scalar(epsilon_L, "epsilon_L"), eR, _scalar(k0dx, "k0dx"), bcs, kas)
     N = nx * ny
     c, arrs, iarrs = _channels_raw(prob, N)
-    mx = np.arange(1, nx + 1)[:, None]
-    my = np.arange(1, ny + 1)[None, :]
-    if bcs[0] == L.RGF_BC_BLOCH:
-        def fun_phi(kx, ky):
-            return np.exp(1j * mx * kx) * (np.exp(1j * my * ky) / math.sqrt(nx * ny))
-    else:
-        def fun_phi(kx, ky):
-            return np.sin(mx * kx) * (np.sin(my * ky) * math.sqrt(4 / ((nx + 1) * (ny + 1))))
+    fun_phi = _profile_fn(prob, N, 3, bcs[0] == L.RGF_BC_BLOCH, (nx, ny))
     ch = {"kxdx": arrs["kt1"].reshape((nx, ny), order="F").copy(),
           "kydx": arrs["kt2"].reshape((nx, ny), order="F").copy(), "fun_phi": fun_phi}
     for s in ("L", "R"):

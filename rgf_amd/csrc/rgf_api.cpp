@@ -1195,6 +1195,30 @@ rgf_status rgf_setup_channels(const rgf_problem* p, rgf_channels* out) {
     });
 }
 
+rgf_status rgf_transverse_profiles(const rgf_problem* p, int64_t count, const double* k1, const double* k2,
+                                   double* phi_out) {
+    return guard(nullptr, [&] {
+        validate_problem(p);
+        if (count < 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "count must be non-negative");
+        if (count > 0 && (!k1 || !phi_out || (p->ndim == 3 && !k2)))
+            throw Error(RGF_ERR_INVALID_ARGUMENT, "wavenumber or output array is NULL");
+        const int64_t n1 = p->dims[0], n2 = p->ndim == 3 ? p->dims[1] : 1;
+        const int b1 = p->bc[0], b2 = p->ndim == 3 ? p->bc[1] : RGF_BC_DIRICHLET;
+        const double s = mode_norm(n1, b1) * (p->ndim == 3 ? mode_norm(n2, b2) : 1.0);
+        std::vector<cd> f2((size_t)n2);
+        for (int64_t c = 0; c < count; ++c) {
+            for (int64_t y = 0; y < n2; ++y) f2[y] = p->ndim == 3 ? mode_shape(b2, y, k2[c]) * s : cd(s, 0);
+            double* col = phi_out + 2 * c * n1 * n2;
+            for (int64_t y = 0; y < n2; ++y)
+                for (int64_t x = 0; x < n1; ++x) {
+                    const cd v = mode_shape(b1, x, k1[c]) * f2[y];  // x fast (setup_channels_3D.m:118,123)
+                    col[2 * (x + n1 * y)] = v.real();
+                    col[2 * (x + n1 * y) + 1] = v.imag();
+                }
+        }
+    });
+}
+
 rgf_status rgf_build_laplacian_1d(int64_t n, int32_t bc, double ka, double* A) {
     return guard(nullptr, [&] {
         if (n < 0) throw Error(RGF_ERR_INVALID_ARGUMENT, "The number of sites n must be a non-negative integer scalar");

@@ -46,30 +46,30 @@ static void transverse_k(int64_t n, int bc, double ka, std::vector<double>& k) {
     }
 }
 
-// The n x n mode table f[m][a] = phi_a(m) (row-major) of one direction
-// (setup_channels.m:133-156).
+// phi(m) = mode_norm(n, bc) * mode_shape(bc, m, k): the mode with
+// wavenumber k at site m (0-based) of one direction of n sites
+// (channels.fun_phi, setup_channels.m:121-156); Neumann's k = 0 mode is the
+// constant 1/sqrt(n).
+double mode_norm(int64_t n, int bc) {
+    if (bc == RGF_BC_BLOCH) return 1.0 / std::sqrt((double)n);
+    if (bc == RGF_BC_DIRICHLET) return std::sqrt(2.0 / (n + 1));
+    if (bc == RGF_BC_NEUMANN) return std::sqrt(2.0 / n);
+    return std::sqrt(2.0 / (n + 0.5));  // Dirichlet-Neumann, Neumann-Dirichlet
+}
+
+cd mode_shape(int bc, int64_t m, double k) {
+    if (bc == RGF_BC_BLOCH) return std::polar(1.0, (double)(m + 1) * k);
+    if (bc == RGF_BC_DIRICHLET || bc == RGF_BC_DIRICHLET_NEUMANN) return std::sin((double)(m + 1) * k);
+    if (bc == RGF_BC_NEUMANN) return std::cos((m + 0.5) * k) - (k == 0 ? (1 - std::sqrt(0.5)) : 0.0);
+    return std::cos((m + 0.5) * k);  // Neumann-Dirichlet
+}
+
+// The n x n mode table f[m][a] = phi_a(m) (row-major) of one direction.
 static void transverse_modes(int64_t n, int bc, const std::vector<double>& k, std::vector<cd>& f) {
     f.assign((size_t)n * n, cd(0, 0));
-    if (bc == RGF_BC_BLOCH) {
-        const double s = 1.0 / std::sqrt((double)n);
-        for (int64_t m = 0; m < n; ++m)
-            for (int64_t a = 0; a < n; ++a) f[m * n + a] = std::polar(s, (double)(m + 1) * k[a]);
-    } else if (bc == RGF_BC_DIRICHLET) {
-        const double s = std::sqrt(2.0 / (n + 1));
-        for (int64_t m = 0; m < n; ++m)
-            for (int64_t a = 0; a < n; ++a) f[m * n + a] = std::sin((double)(m + 1) * k[a]) * s;
-    } else if (bc == RGF_BC_NEUMANN) {
-        const double s = std::sqrt(2.0 / n);
-        for (int64_t m = 0; m < n; ++m)
-            for (int64_t a = 0; a < n; ++a)
-                f[m * n + a] = (std::cos((m + 0.5) * k[a]) - (k[a] == 0 ? (1 - std::sqrt(0.5)) : 0.0)) * s;
-    } else {  // DN / ND
-        const double s = std::sqrt(2.0 / (n + 0.5));
-        for (int64_t m = 0; m < n; ++m)
-            for (int64_t a = 0; a < n; ++a)
-                f[m * n + a] = (bc == RGF_BC_DIRICHLET_NEUMANN) ? std::sin((double)(m + 1) * k[a]) * s
-                                                                : std::cos((m + 0.5) * k[a]) * s;
-    }
+    const double s = mode_norm(n, bc);
+    for (int64_t m = 0; m < n; ++m)
+        for (int64_t a = 0; a < n; ++a) f[m * n + a] = mode_shape(bc, m, k[a]) * s;
 }
 
 void build_modes(Channels& ch) {

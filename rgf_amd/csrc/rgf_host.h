@@ -43,6 +43,8 @@ struct Selection {                  // 0-based channel indices, user order
 };
 
 cd kx_from_z(cd z);
+double mode_norm(int64_t n, int bc);
+cd mode_shape(int bc, int64_t m, double k);
 void validate_problem(const rgf_problem* p);
 // modes = false leaves the transverse mode tables f1, f2 empty (the channel
 // counts, wavenumbers and lead dispersion do not need them); build_modes

@@ -3,6 +3,9 @@
  *
  *   [smatrix, info] = cal_smatrix_RGF_mex(syst, out, in [, opts])
  *   [T, dT_deps]     = cal_smatrix_RGF_mex('gradient', syst, out, in [, opts])
+ *   channels         = cal_smatrix_RGF_mex('channels', syst)
+ *   phi              = cal_smatrix_RGF_mex('phi', syst, kydx)          (2D)
+ *   phi              = cal_smatrix_RGF_mex('phi', syst, kxdx, kydx)    (3D)
  *
  * It reproduces the calling convention of cal_smatrix_RGF.m:1 and
  * cal_smatrix_RGF_3D.m:1 of chiaweihsu/RGF: `syst` is the same struct
@@ -10,9 +13,15 @@
  * and `in` are cells {'L','R'} or structs with ind_out_L/ind_out_R and
  * ind_in_L/ind_in_R.  A 3D epsilon (or a syst with xyBC) selects the 3D
  * solver.  `opts` may hold precision ('fp64' | 'fp32_refined'), refine_tol, refine_s_tol,
- * max_refine_iter and checkpoint_every.  The `channels` output of the
- * reference is produced by the thin wrappers cal_smatrix_RGF.m /
- * cal_smatrix_RGF_3D.m next to this file, which call setup_channels.
+ * max_refine_iter and checkpoint_every.  'channels' returns the struct of
+ * setup_channels.m:30-82 / setup_channels_3D.m:30-64 (kydx, or kxdx and
+ * kydx; L and R with kxdx or kzdx, ind_prop, N_prop, sqrt_mu and, in 2D
+ * where defined, ind_prop_conj) from rgf_setup_channels, with an empty
+ * fun_phi field; 'phi' evaluates the transverse profiles fun_phi returns
+ * (rgf_transverse_profiles) for the transverse size and boundary condition
+ * of syst (its epsilon may have no slices).  The wrappers cal_smatrix_RGF.m /
+ * cal_smatrix_RGF_3D.m next to this file assemble the reference's `channels`
+ * output from the two.  'channels' and 'phi' use no GPU.
  *
  * The gateway only marshals: every check and every number comes from
  * librgf.  Device resources live in one handle created on first use and
@@ -243,13 +252,165 @@ static void get_opts(const mxArray* s, rgf_options* o, int32_t* every) {
     if ((f = mxGetField(s, 0, "checkpoint_every"))) *every = (int32_t)mxGetScalar(f);
 }
 
+/* n complex values (interleaved) as an r x c array; real when every
+ * imaginary part is zero, which is how MATLAB stores such results */
+static mxArray* complex_array(mwSize r, mwSize c, const double* z) {
+    const size_t n = (size_t)r * c;
+    int cx = 0;
+    for (size_t k = 0; k < n; ++k) cx |= z[2 * k + 1] != 0.0;
+    mxArray* a = mxCreateDoubleMatrix(r, c, cx ? mxCOMPLEX : mxREAL);
+    if (cx) {
+        mxComplexDouble* d = mxGetComplexDoubles(a);
+        for (size_t k = 0; k < n; ++k) {
+            d[k].real = z[2 * k];
+            d[k].imag = z[2 * k + 1];
+        }
+    } else {
+        double* d = mxGetDoubles(a);
+        for (size_t k = 0; k < n; ++k) d[k] = z[2 * k];
+    }
+    return a;
+}
+
+static mxArray* real_array(mwSize r, mwSize c, const double* v) {
+    mxArray* a = mxCreateDoubleMatrix(r, c, mxREAL);
+    if (r > 0 && c > 0) memcpy(mxGetDoubles(a), v, sizeof(double) * r * c);
+    return a;
+}
+
+static mxArray* index_row(int32_t n, const int32_t* ind) {
+    mxArray* a = mxCreateDoubleMatrix(1, n, mxREAL);
+    double* d = mxGetDoubles(a);
+    for (int32_t k = 0; k < n; ++k) d[k] = ind[k];
+    return a;
+}
+
+/* setup_longitudinal's struct (setup_channels.m:179-236,
+ * setup_channels_3D.m:146-188); field order as the reference builds it */
+static mxArray* side_struct(const rgf_problem* p, const double* kl, const int32_t* ip, int32_t np,
+                            const double* mu, const int32_t* conj, int has_conj) {
+    const int three = p->ndim == 3;
+    const char* f2[] = {"kxdx", "ind_prop", "N_prop", "sqrt_mu", "ind_prop_conj"};
+    const char* f3[] = {"kzdx", "ind_prop", "N_prop", "sqrt_mu"};
+    mxArray* s = mxCreateStructMatrix(1, 1, three ? 4 : (has_conj ? 5 : 4), three ? f3 : f2);
+    const mwSize r = three ? (mwSize)p->dims[0] : 1, c = three ? (mwSize)p->dims[1] : (mwSize)p->dims[0];
+    mxSetField(s, 0, three ? "kzdx" : "kxdx", complex_array(r, c, kl));
+    mxSetField(s, 0, "ind_prop", index_row(np, ip));
+    mxSetField(s, 0, "N_prop", mxCreateDoubleScalar(np));
+    mxSetField(s, 0, "sqrt_mu", complex_array(1, np, mu));
+    if (!three && has_conj) mxSetField(s, 0, "ind_prop_conj", index_row(np, conj));
+    return s;
+}
+
+/* cal_smatrix_RGF_mex('channels', syst) */
+static mxArray* channels_struct(const rgf_problem* p) {
+    const int64_t N = p->dims[0] * (p->ndim == 3 ? p->dims[1] : 1);
+    double* kt1 = (double*)mxMalloc(sizeof(double) * N);
+    double* kt2 = (double*)mxMalloc(sizeof(double) * N);
+    double* kl[2];
+    double* mu[2];
+    int32_t* ip[2];
+    int32_t* cj[2];
+    for (int k = 0; k < 2; ++k) {
+        kl[k] = (double*)mxMalloc(sizeof(double) * 2 * N);
+        mu[k] = (double*)mxMalloc(sizeof(double) * 2 * N);
+        ip[k] = (int32_t*)mxMalloc(sizeof(int32_t) * N);
+        cj[k] = (int32_t*)mxMalloc(sizeof(int32_t) * N);
+    }
+    rgf_channels ch = {kt1, kt2, kl[0], kl[1], ip[0], ip[1], mu[0], mu[1], cj[0], cj[1], 0, 0, 0};
+    check(rgf_setup_channels(p, &ch), NULL);
+    mxArray* s;
+    if (p->ndim == 3) {
+        const char* f[] = {"kxdx", "kydx", "fun_phi", "L", "R"};
+        s = mxCreateStructMatrix(1, 1, 5, f);
+        mxSetField(s, 0, "kxdx", real_array((mwSize)p->dims[0], (mwSize)p->dims[1], kt1));
+        mxSetField(s, 0, "kydx", real_array((mwSize)p->dims[0], (mwSize)p->dims[1], kt2));
+    } else {
+        const char* f[] = {"kydx", "fun_phi", "L", "R"};
+        s = mxCreateStructMatrix(1, 1, 4, f);
+        mxSetField(s, 0, "kydx", real_array(1, (mwSize)N, kt1));
+    }
+    mxSetField(s, 0, "fun_phi", mxCreateDoubleMatrix(0, 0, mxREAL));
+    mxSetField(s, 0, "L", side_struct(p, kl[0], ip[0], ch.n_prop_L, mu[0], cj[0], ch.has_conj));
+    mxSetField(s, 0, "R", side_struct(p, kl[1], ip[1], ch.n_prop_R, mu[1], cj[1], ch.has_conj));
+    for (int k = 0; k < 2; ++k) {
+        mxFree(kl[k]);
+        mxFree(mu[k]);
+        mxFree(ip[k]);
+        mxFree(cj[k]);
+    }
+    mxFree(kt1);
+    mxFree(kt2);
+    return s;
+}
+
+static const double* real_vector(const mxArray* a, const char* name, size_t* n) {
+    if (!a || !mxIsDouble(a) || mxIsComplex(a)) {
+        char b[96];
+        snprintf(b, sizeof b, "%s must be a real numeric array", name);
+        fail("RGF:input", b);
+    }
+    *n = mxGetNumberOfElements(a);
+    return mxGetDoubles(a);
+}
+
+/* cal_smatrix_RGF_mex('phi', syst, kydx) / ('phi', syst, kxdx, kydx):
+ * 2D: ny x numel(kydx); 3D: nx x ny (x fast) for scalar wavenumbers, nx x ny x K
+ * for K pairs */
+static mxArray* profiles(const rgf_problem* p, int nrhs, const mxArray* prhs[]) {
+    const int three = p->ndim == 3;
+    if (nrhs != (three ? 4 : 3))
+        fail("RGF:input", three ? "usage: cal_smatrix_RGF_mex('phi', syst, kxdx, kydx)"
+                                : "usage: cal_smatrix_RGF_mex('phi', syst, kydx)");
+    size_t n1, n2 = 0;
+    const double* k1 = real_vector(prhs[2], three ? "kxdx" : "kydx", &n1);
+    const double* k2 = three ? real_vector(prhs[3], "kydx", &n2) : NULL;
+    if (three && n1 != n2) fail("RGF:input", "kxdx and kydx must have the same number of elements");
+    const int64_t N = p->dims[0] * (three ? p->dims[1] : 1);
+    double* z = (double*)mxMalloc(sizeof(double) * 2 * (size_t)N * (n1 ? n1 : 1));
+    check(rgf_transverse_profiles(p, (int64_t)n1, k1, k2, z), NULL);
+    mxArray* a;
+    if (three) {
+        const mwSize d[3] = {(mwSize)p->dims[0], (mwSize)p->dims[1], (mwSize)n1};
+        int cx = 0;
+        for (size_t k = 0; k < (size_t)N * n1; ++k) cx |= z[2 * k + 1] != 0.0;
+        a = mxCreateNumericArray(n1 == 1 ? 2 : 3, d, mxDOUBLE_CLASS, cx ? mxCOMPLEX : mxREAL);
+        if (cx) {
+            mxComplexDouble* o = mxGetComplexDoubles(a);
+            for (size_t k = 0; k < (size_t)N * n1; ++k) {
+                o[k].real = z[2 * k];
+                o[k].imag = z[2 * k + 1];
+            }
+        } else {
+            double* o = mxGetDoubles(a);
+            for (size_t k = 0; k < (size_t)N * n1; ++k) o[k] = z[2 * k];
+        }
+    } else {
+        a = complex_array((mwSize)N, (mwSize)n1, z);
+    }
+    mxFree(z);
+    return a;
+}
+
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     int gradient = nrhs >= 1 && mxIsChar(prhs[0]);
     int a0 = gradient ? 1 : 0;
     if (gradient) {
         char mode[16] = {0};
         mxGetString(prhs[0], mode, sizeof mode);
-        if (strcmp(mode, "gradient")) fail("RGF:input", "the only mode argument is 'gradient'");
+        const int chan = !strcmp(mode, "channels"), phi = !strcmp(mode, "phi");
+        if (chan || phi) {
+            if (nrhs < 2) fail("RGF:input", chan ? "usage: cal_smatrix_RGF_mex('channels', syst)"
+                                                 : "usage: cal_smatrix_RGF_mex('phi', syst, kydx)");
+            rgf_problem p;
+            eps_buf eb;
+            read_syst(prhs[1], &p, &eb);
+            plhs[0] = chan ? channels_struct(&p) : profiles(&p, nrhs, prhs);
+            if (eb.re) mxFree(eb.re);
+            if (eb.im) mxFree(eb.im);
+            return;
+        }
+        if (strcmp(mode, "gradient")) fail("RGF:input", "the mode argument must be 'gradient', 'channels' or 'phi'");
     }
     if (nrhs < a0 + 3) fail("RGF:input", "usage: cal_smatrix_RGF_mex(syst, out, in [, opts])");
     rgf_problem p;
