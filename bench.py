@@ -340,6 +340,7 @@ def bench_spheres(args):
         "complex_3m": int(rgf_amd.api._options("fp32_refined", complex_3m=args.complex_3m).complex_3m),
         "refine_iters": infos[-1]["refine_iters"], "refine_residual": infos[-1]["refine_residual"],
         "unitarity_defect_fp32_refined": unitarity(S32),
+        "block_size": infos[-1]["block_size"],
     }
     if not args.skip_fp64:
         if args.warmup > 0:
@@ -349,7 +350,8 @@ def bench_spheres(args):
         S64, _, i64 = solve("fp64")
         t64 = time.perf_counter() - t1
         rec["fp64"] = {"value": round(L / t64, 3), "unit": "slice-updates/s", "seconds": round(t64, 3),
-                       "roofline": roofline(i64, "fp64"), "unitarity_defect": unitarity(S64)}
+                       "roofline": roofline(i64, "fp64"), "unitarity_defect": unitarity(S64),
+                       "block_size": i64["block_size"]}
         rec["ratio_vs_fp64"] = round(value / (L / t64), 3)
         rec["s_err_vs_fp64"] = float(np.abs(S32 - S64).max())
         rec["s_err_within_bound"] = bool(rec["s_err_vs_fp64"] <= S_ERR_BOUND)

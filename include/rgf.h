@@ -31,8 +31,9 @@
 extern "C" {
 #endif
 
-#define RGF_ABI_VERSION 3  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate;
-                              3: rgf_info.max_multiplier */
+#define RGF_ABI_VERSION 4  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate;
+                              3: rgf_info.max_multiplier;
+                              4: rgf_options.autotune, rgf_info.block_size and autotuned */
 #define RGF_ALL (-1) /* selection count meaning "all propagating channels 1..N_prop" */
 
 typedef enum {
@@ -111,7 +112,13 @@ typedef struct {
                                  plain launches; 2 = every problem on worker contexts */
     int32_t batch_graph_rounds;/* lockstep engine: block-step rounds per captured
                                  graph (0 = 16)                                      */
-    int32_t reserved[1];
+    int32_t autotune;         /* block_size 0: 1 (default) = the slice-inverse block width
+                                 and stream use are timed once per process for each
+                                 (device, N, precision, complex_3m bit 0,
+                                 single_stream) on a synthetic slice matrix (FP32 64
+                                 or 128, FP64 32 or 64; look-ahead streams or one
+                                 stream) and the fastest kept; 0 = fixed (FP32 128,
+                                 FP64 64, look-ahead unless single_stream)           */
     double refine_s_tol;      /* FP32_REFINED, > 0: also stop once the estimated max
                                  error of the S entries is below this (see
                                  rgf_info.refine_s_err_estimate); 0 = off (default)  */
@@ -144,6 +151,8 @@ typedef struct {
        growth the rows outside a diagonal block see; pivoting is confined to
        the block).  The FP64 paths fail with RGF_ERR_NUMERICAL above 1e6. */
     double max_multiplier;
+    int32_t block_size;       /* slice-inverse block width used                         */
+    int32_t autotuned;        /* 1: this call timed the block-width candidates         */
 } rgf_info;
 
 /* Lead channels (setup_channels.m:30-82; setup_channels_3D.m:30-64).

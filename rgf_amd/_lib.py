@@ -53,7 +53,7 @@ class Options(C.Structure):
         ("precision", C.c_int32), ("max_refine_iter", C.c_int32), ("refine_tol", C.c_double),
         ("block_size", C.c_int32), ("complex_3m", C.c_int32), ("profile", C.c_int32),
         ("single_stream", C.c_int32), ("batch_concurrency", C.c_int32), ("batch_engine", C.c_int32),
-        ("batch_graph_rounds", C.c_int32), ("reserved", C.c_int32 * 1),
+        ("batch_graph_rounds", C.c_int32), ("autotune", C.c_int32),
         ("refine_s_tol", C.c_double),
     ]
 
@@ -67,6 +67,7 @@ class Info(C.Structure):
         ("seconds_slices", C.c_double), ("seconds_refine", C.c_double), ("flops_slices", C.c_double),
         ("kernel_seconds", C.c_double), ("kernel_flops", C.c_double), ("kernel_launches", C.c_int64),
         ("refine_s_err_estimate", C.c_double), ("max_multiplier", C.c_double),
+        ("block_size", C.c_int32), ("autotuned", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -276,7 +276,7 @@ def _options(precision="fp64", **kw) -> L.Options:
     if precision not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
     o.precision = _PRECISIONS[precision]
-    known = {f[0] for f in L.Options._fields_} - {"precision", "reserved"}
+    known = {f[0] for f in L.Options._fields_} - {"precision"}
     for k, v in kw.items():
         if k not in known:
             raise ValueError(f"unknown option {k!r}; options are {sorted(known)}")

@@ -293,11 +293,107 @@ std::pair<T*, size_t> splitk_ws(rgf_context* c) {
 
 int default_block(int precision) { return precision == RGF_PRECISION_FP64 ? 64 : 128; }
 
+// ---------------------------------------------------------------- slice-inverse autotuner
+// The slice inverse's block width and stream use, timed once per process for
+// each (device, N, precision, three-product bit, single_stream option) on a
+// synthetic slice matrix of that size (tune_fill): widths FP32 {64, 128},
+// FP64 {32, 64}, each with the look-ahead streams and on one stream (only
+// the latter when the caller asked for one stream).  Each candidate runs
+// three times, interleaved with the others (HIP events on the handle's
+// stream); the first run warms up and the faster of the other two counts.
+// A candidate whose inverse reports a zero pivot is dropped.
+struct SliceCfg {
+    int b = 0;
+    bool lookahead = true;
+};
+struct TuneKey {
+    int dev;
+    int64_t n;
+    int prec, use3m, single;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(dev, n, prec, use3m, single) < std::tie(o.dev, o.n, o.prec, o.use3m, o.single);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, SliceCfg> g_tuned;
+
+template <typename T>
+SliceCfg tune_slice(rgf_context* c, int64_t N, int use3m, const std::vector<SliceCfg>& cand) {
+    hipStream_t s = c->stream;
+    size_t wse = 0;
+    for (const SliceCfg& k : cand) wse = std::max(wse, inverse_workspace_elems<T>(N, k.b));
+    CMat<T> A = dmat<T>(c, "tune_A", N, N);
+    T* ws = dbuf<T>(c, "tune_ws", wse);
+    hipEvent_t e0, e1;
+    RGF_HIP(hipEventCreate(&e0));
+    RGF_HIP(hipEventCreate(&e1));
+    std::vector<double> best(cand.size(), INFINITY);
+    for (int rep = 0; rep < 3; ++rep)
+        for (size_t k = 0; k < cand.size(); ++k) {
+            tune_fill<T>(s, N, A, 17u + (uint32_t)rep);
+            reset_status(c);
+            RGF_HIP(hipEventRecord(e0, s));
+            invert_inplace<T>(s, N, A, cand[k].b, ws, c->d_status, use3m, nullptr,
+                              cand[k].lookahead ? &c->ov : nullptr);
+            RGF_HIP(hipEventRecord(e1, s));
+            int st[2] = {0, 0};
+            RGF_HIP(hipMemcpyAsync(st, c->d_status, sizeof st, hipMemcpyDeviceToHost, s));
+            RGF_HIP(hipStreamSynchronize(s));
+            float ms = 0;
+            RGF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (st[0]) best[k] = -1;  // unusable
+            else if (rep > 0 && best[k] >= 0) best[k] = std::min(best[k], (double)ms);
+        }
+    RGF_HIP(hipEventDestroy(e0));
+    RGF_HIP(hipEventDestroy(e1));
+    release(c, "tune_A");
+    release(c, "tune_ws");
+    reset_status(c);
+    int pick = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
+        if (best[k] >= 0 && (best[pick] < 0 || best[k] < best[pick])) pick = (int)k;
+    if (getenv("RGF_TUNE_VERBOSE")) {
+        fprintf(stderr, "rgf autotune: N=%lld %s%s:", (long long)N, sizeof(T) == 8 ? "fp64" : "fp32",
+                use3m ? " 3m" : "");
+        for (size_t k = 0; k < cand.size(); ++k)
+            fprintf(stderr, " b=%d%s %.3f ms", cand[k].b, cand[k].lookahead ? "" : "/1s", best[k]);
+        fprintf(stderr, " -> b=%d%s\n", cand[pick].b, cand[pick].lookahead ? "" : "/1s");
+    }
+    return best[pick] < 0 ? SliceCfg{} : cand[pick];
+}
+
+// The block width and stream use of the slice inverses of a solve:
+// opt.block_size when set, else the tuned width (opt.autotune) or the fixed
+// default; one stream when opt.single_stream is set or the tuner chose it.
+SliceCfg slice_config(rgf_context* c, int64_t N, int precision, int use3m, const rgf_options& o, Work& w) {
+    SliceCfg cfg{o.block_size > 0 ? o.block_size : default_block(precision), !o.single_stream};
+    if (o.block_size <= 0 && o.autotune && N > 64) {
+        const TuneKey key{c->device, N, precision, use3m, o.single_stream ? 1 : 0};
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it == g_tuned.end()) {
+            std::vector<SliceCfg> cand;
+            for (int b : precision == RGF_PRECISION_FP64 ? std::vector<int>{32, 64} : std::vector<int>{64, 128}) {
+                if (!o.single_stream) cand.push_back({b, true});
+                cand.push_back({b, false});
+            }
+            SliceCfg t = precision == RGF_PRECISION_FP64 ? tune_slice<double>(c, N, 0, cand)
+                                                         : tune_slice<float>(c, N, use3m, cand);
+            it = g_tuned.emplace(key, t.b > 0 ? t : cfg).first;
+            w.tuned = 1;
+        }
+        cfg = it->second;
+    }
+    w.block_size = cfg.b;
+    return cfg;
+}
+
 // ---------------------------------------------------------------- FP64 carry driver
 void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     const int64_t N = w.N, L = w.L;
     const int64_t pi = w.inL.size(), po = w.outL.size();
-    const int b = o.block_size > 0 ? o.block_size : default_block(RGF_PRECISION_FP64);
+    const SliceCfg cfg = slice_config(c, N, RGF_PRECISION_FP64, 0, o, w);
+    const int b = cfg.b;
     hipStream_t s = c->stream;
     CMat<double> G = dmat<double>(c, "G", N, N);
     double* ws = dbuf<double>(c, "inv_ws", inverse_workspace_elems<double>(N, b));
@@ -322,7 +418,7 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
                                    cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<double>(s, N, G, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
-        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer, o.single_stream ? nullptr : &c->ov);
+        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer, cfg.lookahead ? &c->ov : nullptr);
         if (pi) {
             cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             std::swap(X, X2);
@@ -365,10 +461,11 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     // a multiple of 4 (16-byte rows for the FP32 sweep GEMMs; the padding
     // solves H psi = 0 and stays zero)
     const int64_t mL = w.inL.size(), mR = w.inR.size(), m = (mL + mR + 3) / 4 * 4;
-    const int b = o.block_size > 0 ? o.block_size : default_block(RGF_PRECISION_FP32_REFINED);
     // three real products per complex product: bit 0 in the slice inverses,
     // bit 1 in the refinement sweeps
     const int use3m = o.complex_3m & 1, sweep3m = (o.complex_3m >> 1) & 1;
+    const SliceCfg cfg = slice_config(c, N, RGF_PRECISION_FP32_REFINED, use3m, o, w);
+    const int b = cfg.b;
     hipStream_t s = c->stream;
     // factor store: S slices of N x N complex FP32
     size_t freeb = 0, totalb = 0;
@@ -397,8 +494,7 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
             copy_mat<float>(s, N, N, prev, Fn(n));
             assemble_slice<float>(s, N, Fn(n), nullptr, nullptr, w.epsR, w.k2, &G0R32, w.A0);
         }
-        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer,
-                              o.single_stream ? nullptr : &c->ov);
+        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer, cfg.lookahead ? &c->ov : nullptr);
     }
     RGF_HIP(hipStreamSynchronize(s));
     w.t_slices = now() - t0;
@@ -593,9 +689,11 @@ void fill_info(rgf_info* info, const Channels& ch, const Selection& sel) {
 //   psi_n = z_n + g_n psi_{n+1}    (psi_{L+1} = z_{L+1}: forward fields)
 //   phi_n = phi_{n+1} g_n           (phi_{L+1} = t^H C_R g_{L+1}: adjoint rows)
 // and dT/d eps_{m,n} from g = -2i k^2 sum_j psi_n[m,j] phi_n[j,m].
-void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<double>& gre, std::vector<double>& gim) {
+void run_gradient(rgf_context* c, Work& w, const rgf_options& o, int every, double& T, std::vector<double>& gre,
+                  std::vector<double>& gim) {
     const int64_t N = w.N, L = w.L, m = w.inL.size(), no = w.outR.size();
-    const int b = default_block(RGF_PRECISION_FP64);
+    const SliceCfg cfg = slice_config(c, N, RGF_PRECISION_FP64, 0, o, w);
+    const int b = cfg.b;
     hipStream_t s = c->stream;
     const int64_t C = (every <= 0 || every > L + 1) ? L + 2 : every;
     const int64_t nck = L / C + 1;  // checkpoints at n = 0, C, 2C, ... <= L
@@ -621,7 +719,7 @@ void run_gradient(rgf_context* c, Work& w, int every, double& T, std::vector<dou
             copy_mat<double>(s, N, N, prev, out);
             assemble_slice<double>(s, N, out, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
         }
-        invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer, &c->ov);
+        invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer, cfg.lookahead ? &c->ov : nullptr);
     };
     reset_status(c);
     RGF_HIP(hipStreamSynchronize(s));
@@ -862,7 +960,7 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
     double T = 0;
     std::vector<double> gre, gim;
     if (o.precision == RGF_PRECISION_FP64)
-        run_gradient(c, w, every, T, gre, gim);
+        run_gradient(c, w, o, every, T, gre, gim);
     else
         run_gradient_refined(c, w, o, T, gre, gim);
     if (T_out) *T_out = T;
@@ -883,6 +981,8 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
         info->refine_residual = w.refine_residual;
         info->flops_slices = w.flops;
         info->max_multiplier = w.max_multiplier;
+        info->block_size = w.block_size;
+        info->autotuned = w.tuned;
         if (timer) {
             info->kernel_seconds = timer->seconds();
             info->kernel_flops = timer->flops;
@@ -987,6 +1087,8 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
         info->refine_residual = w.refine_residual;
         info->refine_s_err_estimate = w.refine_s_err;
         info->max_multiplier = w.max_multiplier;
+        info->block_size = w.block_size;
+        info->autotuned = w.tuned;
         info->seconds_setup = w.t_setup;
         info->seconds_slices = w.t_slices;
         info->seconds_refine = w.t_refine;
@@ -1070,6 +1172,7 @@ void rgf_default_options(rgf_options* o) {
     o->refine_tol = 1e-13;
     o->block_size = 0;
     o->complex_3m = 3;  // three real products in the slice inverses and the refinement sweeps
+    o->autotune = 1;
 }
 
 rgf_status rgf_create(rgf_handle* h, int32_t device) {
@@ -1285,6 +1388,19 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
             cost[(size_t)q] = n * n * (double)std::max<int64_t>(p[q].dims[nd - 1], 1);
         }
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+        // block widths tuned here, one size at a time, before the workers
+        // share the GPU (timings taken beside other solves would mislead)
+        if (nw > 1)
+            for (int32_t q : order) {
+                if ((p[q].ndim != 2 && p[q].ndim != 3) ||
+                    (o.precision != RGF_PRECISION_FP64 && o.precision != RGF_PRECISION_FP32_REFINED))
+                    continue;
+                const int64_t N = p[q].dims[0] * (p[q].ndim == 3 ? p[q].dims[1] : 1);
+                if (N <= 0) continue;
+                RGF_HIP(hipSetDevice(h->device));
+                Work tw;
+                slice_config(h, N, o.precision, o.precision == RGF_PRECISION_FP64 ? 0 : (o.complex_3m & 1), o, tw);
+            }
         std::atomic<int32_t> next{0};
         auto run = [&](int wi) {
             rgf_context* c = h->workers[(size_t)wi];

@@ -547,6 +547,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
             in.refine_iters = hp.iters;
             in.refine_residual = hp.rel;
             in.refine_s_err_estimate = hp.s_err;
+            in.block_size = 64;  // the engine's block step
             in.seconds_setup = t_setup;
             in.seconds_slices = t_slices;
             in.seconds_refine = t_refine;

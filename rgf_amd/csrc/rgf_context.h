@@ -87,6 +87,8 @@ struct Work {
     double refine_residual = 0;
     double refine_s_err = 0;  // estimated S error at the stop (refine_s_tol)
     double max_multiplier = 0;  // largest slice-inverse multiplier
+    int block_size = 0;         // slice-inverse block width used
+    int tuned = 0;              // 1: this solve ran the block-width tuner
     double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
 };
 

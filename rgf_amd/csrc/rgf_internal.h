@@ -116,6 +116,9 @@ void assemble_slice(hipStream_t s, int64_t N, CMat<T> M, const double* eps_re, c
 // Dense helpers.
 template <typename T>
 void fill_zero(hipStream_t s, int64_t rows, int64_t cols, CMat<T> A);
+// a well-conditioned synthetic slice matrix (for timing the slice inverse)
+template <typename T>
+void tune_fill(hipStream_t s, int64_t N, CMat<T> A, uint32_t seed);
 template <typename T>
 void set_identity(hipStream_t s, int64_t N, CMat<T> A);
 template <typename Td, typename Ts>

@@ -2213,6 +2213,33 @@ void set_identity(hipStream_t s, int64_t N, CMat<T> A) {
     hipLaunchKernelGGL((fill_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, N, A.re, A.im, A.ld, 1);
 }
 
+// A synthetic slice matrix for timing the slice inverse: diagonal 4 (the
+// stencil's), off-diagonal entries with independent real and imaginary parts
+// uniform in [-a, a], a = 0.5 / sqrt(N) (spectral norm of the off-diagonal
+// part about 0.6), from a hash of (seed, i, j).
+template <typename T>
+__global__ void tune_fill_kernel(int64_t N, T* re, T* im, int64_t ld, uint32_t seed) {
+    const double a = 0.5 / sqrt((double)N);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * N;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / N, j = e % N;
+        uint64_t h = (uint64_t)e * 0x9E3779B97F4A7C15ull + seed;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        const double u = (double)(h >> 40) * (1.0 / 16777216.0), v = (double)(h & 0xFFFFFF) * (1.0 / 16777216.0);
+        re[i * ld + j] = (T)((2 * u - 1) * a + (i == j ? 4.0 : 0.0));
+        im[i * ld + j] = (T)((2 * v - 1) * a);
+    }
+}
+
+template <typename T>
+void tune_fill(hipStream_t s, int64_t N, CMat<T> A, uint32_t seed) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL((tune_fill_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, A.re, A.im, A.ld, seed);
+    RGF_HIP(hipGetLastError());
+}
+
 template <typename Td, typename Ts>
 __global__ void convert_kernel(int64_t rows, int64_t cols, const Ts* sre, const Ts* sim, int64_t lds, Td* dre,
                                Td* dim, int64_t ldd, double scale) {
@@ -2663,6 +2690,7 @@ void zero_words(hipStream_t s, int* p, int n) {
                                     const CMat<T>*, const DevCSR&, const CMat<T>*);                           \
     template void fill_zero<T>(hipStream_t, int64_t, int64_t, CMat<T>);                                       \
     template void set_identity<T>(hipStream_t, int64_t, CMat<T>);                                             \
+    template void tune_fill<T>(hipStream_t, int64_t, CMat<T>, uint32_t);                                      \
     template void copy_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>);                               \
     template void accumulate<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<double>, double);                \
     template void add_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>);

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
     assert declared == set(L.SIGNATURES), declared ^ set(L.SIGNATURES)
-    assert lib.rgf_abi_version() == 3
+    assert lib.rgf_abi_version() == 4
 
 
 def test_status_strings_and_default_options():
@@ -34,7 +34,35 @@ def test_status_strings_and_default_options():
     o_ = L.Options()
     lib.rgf_default_options(C.byref(o_))
     assert o_.precision == 0 and o_.refine_tol == 1e-13 and o_.max_refine_iter == 6
-    assert o_.refine_s_tol == 0 and o_.complex_3m == 3
+    assert o_.refine_s_tol == 0 and o_.complex_3m == 3 and o_.autotune == 1
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of rgf_options / rgf_info / rgf_problem /
+    rgf_selection / rgf_channels have the C header's size and field offsets
+    (a C program built with gcc prints them)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"rgf_options": L.Options, "rgf_info": L.Info, "rgf_problem": L.Problem,
+               "rgf_selection": L.Selection, "rgf_channels": L.Channels}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "rgf.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'    printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                   text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
 
 
 BCS = ["periodic", "Dirichlet", "Neumann", "DirichletNeumann", "NeumannDirichlet", "PEC", "pmc", 0.9, -2.5]

@@ -42,7 +42,7 @@ def _well_posed(rng, n):
 
 
 @pytest.mark.parametrize("n,b", [(1, 0), (5, 0), (64, 0), (100, 16), (200, 32), (333, 64), (700, 0), (129, 64),
-                                 (130, 64), (192, 64)])
+                                 (130, 64), (192, 64), (1000, 32)])
 def test_slice_inverse_fp64(n, b):
     rng = np.random.default_rng(n + b)
     A = _well_posed(rng, n)
