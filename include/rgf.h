@@ -33,7 +33,8 @@ extern "C" {
 
 #define RGF_ABI_VERSION 4  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate;
                               3: rgf_info.max_multiplier;
-                              4: rgf_options.autotune, rgf_info.block_size and autotuned */
+                              4: rgf_options.autotune and factor_slots, rgf_info.block_size,
+                                 autotuned, factor_slices_resident and factor_slices_rebuilt */
 #define RGF_ALL (-1) /* selection count meaning "all propagating channels 1..N_prop" */
 
 typedef enum {
@@ -122,6 +123,13 @@ typedef struct {
     double refine_s_tol;      /* FP32_REFINED, > 0: also stop once the estimated max
                                  error of the S entries is below this (see
                                  rgf_info.refine_s_err_estimate); 0 = off (default)  */
+    int32_t factor_slots;     /* FP32_REFINED: at most this many N x N FP32 slices of
+                                 device memory for the slice inverses (0 = as many as
+                                 free memory allows).  With fewer than L + 2, the
+                                 first R inverses stay resident and the others are
+                                 rebuilt in segments by each refinement sweep from
+                                 stored checkpoints (memory O((R + L/C + C) N^2)
+                                 instead of O(L N^2), more inverses computed)     */
 } rgf_options;
 
 typedef struct {
@@ -153,6 +161,11 @@ typedef struct {
     double max_multiplier;
     int32_t block_size;       /* slice-inverse block width used                         */
     int32_t autotuned;        /* 1: this call timed the block-width candidates         */
+    /* FP32_REFINED: slice inverses kept in device memory (L + 2 when all fit;
+       fewer when the rest were rebuilt by the sweeps, see factor_slots) and
+       how many the sweeps rebuilt */
+    int64_t factor_slices_resident;
+    int64_t factor_slices_rebuilt;
 } rgf_info;
 
 /* Lead channels (setup_channels.m:30-82; setup_channels_3D.m:30-64).

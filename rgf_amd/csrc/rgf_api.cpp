@@ -454,8 +454,103 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
     }
 }
 
+// ---------------------------------------------------------------- mixed-path factor store
+// The FP32 slice inverses g_0 .. g_{S-1} of the mixed path.  When all S fit
+// in device memory they stay resident.  Otherwise g_0 .. g_{R-1} stay
+// resident and the slices from R on are grouped in segments of C slices that
+// the refinement sweeps rebuild when they reach them: segment k (slices
+// R + kC .. R + (k+1)C - 1) is rebuilt by the Dyson steps of the first pass
+// from the slice before it, g_{R-1} (resident) for k = 0 and a stored
+// checkpoint for k > 0, with the same kernels and inputs, so the rebuilt
+// inverses are the same numbers.  Device memory: R + (segments - 1) + C
+// slices instead of S.  A forward sweep rebuilds every segment once and a
+// backward sweep every segment but the one it starts in.
+struct FactorStore {
+    rgf_context* c = nullptr;
+    const Work* w = nullptr;
+    int64_t N = 0, S = 0, R = 0, C = 1, nseg = 0;
+    int b = 0, use3m = 0;
+    const Overlap* ov = nullptr;
+    float* res = nullptr;  // R slots: g_0 .. g_{R-1}
+    float* ck = nullptr;   // nseg - 1 slots: ck[k - 1] = g_{R + kC - 1}
+    float* seg = nullptr;  // C slots: the segment `cur`
+    float* ws = nullptr;   // slice-inverse workspace
+    CMat<float> G0R32;     // the right lead's G0 (assembly of slice S - 1)
+    int64_t cur = -1;
+    int64_t rebuilt = 0;   // slices rebuilt by the sweeps
+
+    bool hybrid() const { return R < S; }
+    CMat<float> slot(float* base, int64_t k) const {
+        float* p = base + (size_t)k * N * N * 2;
+        return CMat<float>(p, p + (size_t)N * N, N);
+    }
+    // g_n = (D_n - g_{n-1})^-1 into out (the first pass's step)
+    void step(int64_t n, CMat<float> prev, CMat<float> out, KernelTimer* timer) const {
+        hipStream_t s = c->stream;
+        const int64_t L = S - 2;
+        if (n <= L) {
+            assemble_slice<float>(s, N, out, w->eps_re + (n - 1) * N, w->eps_im ? w->eps_im + (n - 1) * N : nullptr,
+                                  cd(0, 0), w->k2, nullptr, w->A0, &prev);
+        } else {
+            copy_mat<float>(s, N, N, prev, out);
+            assemble_slice<float>(s, N, out, nullptr, nullptr, w->epsR, w->k2, &G0R32, w->A0);
+        }
+        invert_inplace<float>(s, N, out, b, ws, c->d_status, use3m, timer, ov);
+    }
+    // segment k into the segment slots; in the first pass, also store the
+    // checkpoint of segment k + 1
+    void build(int64_t k, bool first, KernelTimer* timer) {
+        const int64_t n0 = R + k * C, n1 = std::min(n0 + C, S);
+        for (int64_t n = n0; n < n1; ++n)
+            step(n, n > n0 ? slot(seg, n - 1 - n0) : (k == 0 ? slot(res, R - 1) : slot(ck, k - 1)), slot(seg, n - n0),
+                 timer);
+        if (first && k + 1 < nseg) copy_mat<float>(c->stream, N, N, slot(seg, n1 - 1 - n0), slot(ck, k));
+        if (!first) rebuilt += n1 - n0;
+        cur = k;
+    }
+    // g_n, rebuilding its segment first when needed
+    CMat<float> at(int64_t n) {
+        if (n < R) return slot(res, n);
+        const int64_t k = (n - R) / C;
+        if (k != cur) build(k, false, nullptr);
+        return slot(seg, n - R - k * C);
+    }
+};
+
+// The resident slices R and segment length C for at most K slices of
+// device memory (R + (segments - 1) + C <= K): the most resident slices,
+// then the longest segments.  Returns false when even R = 1 does not fit.
+bool plan_factor_store(int64_t S, int64_t K, int64_t& R, int64_t& C) {
+    if (K >= S) {
+        R = S;
+        C = 1;
+        return true;
+    }
+    R = 0;
+    C = 1;
+    for (int64_t c = 1; c + 1 <= K; ++c) {
+        // the largest r with r + ceil((S - r) / c) - 1 + c <= K (the left side
+        // does not decrease with r)
+        int64_t lo = 1, hi = S - 1, best = 0;
+        while (lo <= hi) {
+            const int64_t r = (lo + hi) / 2;
+            if (r + (S - r + c - 1) / c - 1 + c <= K) {
+                best = r;
+                lo = r + 1;
+            } else {
+                hi = r - 1;
+            }
+        }
+        if (best > R || (best == R && best > 0)) {
+            R = best;
+            C = c;
+        }
+    }
+    return R >= 1;
+}
+
 // ---------------------------------------------------------------- FP32 + refinement driver
-void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
+void run_refined(rgf_context* c, Work& w, const rgf_options& o, FactorStore* keep = nullptr, double extra_bytes = 0) {
     const int64_t N = w.N, L = w.L, S = L + 2;
     // right-hand sides: mL left columns, mR right columns, zero columns up to
     // a multiple of 4 (16-byte rows for the FP32 sweep GEMMs; the padding
@@ -467,41 +562,59 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     const SliceCfg cfg = slice_config(c, N, RGF_PRECISION_FP32_REFINED, use3m, o, w);
     const int b = cfg.b;
     hipStream_t s = c->stream;
-    // factor store: S slices of N x N complex FP32
+    // factor store: up to S slices of N x N complex FP32, beside the FP64
+    // right-hand sides, residuals and FP32 corrections (and extra_bytes the
+    // caller needs after the solve)
     size_t freeb = 0, totalb = 0;
     RGF_HIP(hipMemGetInfo(&freeb, &totalb));
-    const double need = (double)S * N * N * 8 + (double)S * N * m * (16 + 16 + 8) + (double)N * N * 8;
+    const double slice_bytes = (double)N * N * 8;
+    const double fixed = (double)S * N * m * (16 + 16 + 8) + slice_bytes +
+                         (double)inverse_workspace_elems<float>(N, b) * sizeof(float) + extra_bytes;
     // buffers kept from an earlier solve on this handle are reused (or freed
     // and grown), so what they hold counts as available
-    const double avail = (double)freeb + (double)held_bytes(c, {"factors", "psi", "res", "delta", "G0R32"});
-    if (need > 0.92 * avail)
+    const double avail =
+        (double)freeb + (double)held_bytes(c, {"factors", "factor_ck", "factor_seg", "psi", "res", "delta", "G0R32",
+                                               "inv_ws32", "adj_y", "adj_r", "adj_d"});
+    int64_t K = (int64_t)std::max(0.0, std::floor((0.92 * avail - fixed) / slice_bytes));
+    if (o.factor_slots > 0) K = std::min<int64_t>(K, o.factor_slots);
+    FactorStore local;
+    FactorStore& fs = keep ? *keep : local;
+    fs = FactorStore();
+    if (!plan_factor_store(S, K, fs.R, fs.C))
         throw Error(RGF_ERR_OUT_OF_MEMORY, "mixed-precision factor store does not fit in device memory");
-    float* F = dbuf<float>(c, "factors", (size_t)S * N * N * 2);
-    auto Fn = [&](int64_t n) { return CMat<float>(F + (size_t)n * N * N * 2, F + (size_t)n * N * N * 2 + (size_t)N * N, N); };
-    float* ws = dbuf<float>(c, "inv_ws32", inverse_workspace_elems<float>(N, b));
-    CMat<float> G0R32 = dmat<float>(c, "G0R32", N, N);
-    convert<float, double>(s, N, N, w.G0L, Fn(0), 1.0);
-    convert<float, double>(s, N, N, w.G0R, G0R32, 1.0);
+    fs.c = c;
+    fs.w = &w;
+    fs.N = N;
+    fs.S = S;
+    fs.nseg = fs.hybrid() ? (S - fs.R + fs.C - 1) / fs.C : 0;
+    fs.b = b;
+    fs.use3m = use3m;
+    fs.ov = cfg.lookahead ? &c->ov : nullptr;
+    fs.res = dbuf<float>(c, "factors", (size_t)fs.R * N * N * 2);
+    if (fs.hybrid()) {
+        fs.ck = fs.nseg > 1 ? dbuf<float>(c, "factor_ck", (size_t)(fs.nseg - 1) * N * N * 2) : nullptr;
+        fs.seg = dbuf<float>(c, "factor_seg", (size_t)fs.C * N * N * 2);
+    } else {
+        release(c, "factor_ck");
+        release(c, "factor_seg");
+    }
+    w.factors_resident = fs.R;
+    fs.ws = dbuf<float>(c, "inv_ws32", inverse_workspace_elems<float>(N, b));
+    fs.G0R32 = dmat<float>(c, "G0R32", N, N);
+    convert<float, double>(s, N, N, w.G0L, fs.slot(fs.res, 0), 1.0);
+    convert<float, double>(s, N, N, w.G0R, fs.G0R32, 1.0);
     reset_status(c);
     RGF_HIP(hipStreamSynchronize(s));
     const double t0 = now();
-    for (int64_t n = 1; n <= L + 1; ++n) {
-        const CMat<float> prev = Fn(n - 1);
-        if (n <= L) {
-            assemble_slice<float>(s, N, Fn(n), w.eps_re + (n - 1) * N, w.eps_im ? w.eps_im + (n - 1) * N : nullptr,
-                                  cd(0, 0), w.k2, nullptr, w.A0, &prev);
-        } else {
-            copy_mat<float>(s, N, N, prev, Fn(n));
-            assemble_slice<float>(s, N, Fn(n), nullptr, nullptr, w.epsR, w.k2, &G0R32, w.A0);
-        }
-        invert_inplace<float>(s, N, Fn(n), b, ws, c->d_status, use3m, w.timer, cfg.lookahead ? &c->ov : nullptr);
-    }
+    for (int64_t n = 1; n < fs.R; ++n) fs.step(n, fs.slot(fs.res, n - 1), fs.slot(fs.res, n), w.timer);
+    for (int64_t k = 0; k < fs.nseg; ++k) fs.build(k, true, w.timer);
     RGF_HIP(hipStreamSynchronize(s));
     w.t_slices = now() - t0;
     w.max_multiplier = check_status(c, false, false);
     const double Nd = (double)N;
     w.flops = (double)(L + 1) * 8 * Nd * Nd * Nd * (use3m ? 0.75 : 1.0);
-    release(c, "G0R32");
+    if (!fs.hybrid()) release(c, "G0R32");
+    auto Fn = [&](int64_t n) { return fs.at(n); };
 
     // refinement: psi (FP64), r (FP64), d (FP32), slices of N x m
     const double t1 = now();
@@ -612,6 +725,8 @@ void run_refined(rgf_context* c, Work& w, const rgf_options& o) {
     w.refine_iters = it;
     w.refine_residual = rel;
     w.flops += ((double)it * (2.0 * S - 1) * m - (double)skipped_cols) * 8 * Nd * Nd * (sweep3m ? 0.75 : 1.0);
+    w.factors_rebuilt = fs.rebuilt;
+    w.flops += (double)fs.rebuilt * 8 * Nd * Nd * Nd * (use3m ? 0.75 : 1.0);
     // S blocks from the boundary slices: C_L psi_0, C_R psi_{L+1}
     const int64_t po = w.outL.size(), qo = w.outR.size();
     if (po) {
@@ -823,9 +938,13 @@ HostCSR transpose_csr(const HostCSR& a) {
 // contraction of run_gradient: u_i = k^2 sum_j psi_n[i,j] y_n[j,i].
 void run_gradient_refined(rgf_context* c, Work& w, const rgf_options& o, double& T, std::vector<double>& gre,
                           std::vector<double>& gim) {
-    run_refined(c, w, o);
     const int64_t N = w.N, L = w.L, S = L + 2;
     const int64_t mL = (int64_t)w.inL.size(), m = (mL + 3) / 4 * 4, no = (int64_t)w.outR.size();
+    // the factor store is sized with room for the adjoint rows, residuals
+    // and corrections allocated below
+    FactorStore fs;
+    run_refined(c, w, o, &fs, (double)S * N * m * (16 + 16 + 8));
+    const int64_t rebuilt0 = fs.rebuilt;
     hipStream_t s = c->stream;
     const double t0 = now();
     T = 0;
@@ -855,10 +974,7 @@ void run_gradient_refined(rgf_context* c, Work& w, const rgf_options& o, double&
         RGF_HIP(hipStreamSynchronize(s));
     }
     const size_t SE = (size_t)N * m;
-    float* F = (float*)c->bufs["factors"].p;
-    auto Fn = [&](int64_t n) {
-        return CMat<float>(F + (size_t)n * N * N * 2, F + (size_t)n * N * N * 2 + (size_t)N * N, N);
-    };
+    auto Fn = [&](int64_t n) { return fs.at(n); };
     double* y = dbuf<double>(c, "adj_y", 2 * S * SE);
     double* r = dbuf<double>(c, "adj_r", 2 * S * SE);
     float* d = dbuf<float>(c, "adj_d", 2 * S * SE);
@@ -930,6 +1046,9 @@ void run_gradient_refined(rgf_context* c, Work& w, const rgf_options& o, double&
     RGF_HIP(hipStreamSynchronize(s));
     w.t_refine += now() - t0;
     w.refine_iters += it;  // forward and adjoint correction sweeps
+    w.factors_rebuilt = fs.rebuilt;
+    w.flops += (double)(fs.rebuilt - rebuilt0) * 8 * (double)N * N * N * ((o.complex_3m & 1) ? 0.75 : 1.0);
+    if (fs.hybrid()) release(c, "G0R32");
 }
 
 void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, const rgf_options* opt, int every,
@@ -983,6 +1102,8 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
         info->max_multiplier = w.max_multiplier;
         info->block_size = w.block_size;
         info->autotuned = w.tuned;
+        info->factor_slices_resident = w.factors_resident;
+        info->factor_slices_rebuilt = w.factors_rebuilt;
         if (timer) {
             info->kernel_seconds = timer->seconds();
             info->kernel_flops = timer->flops;
@@ -1089,6 +1210,8 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
         info->max_multiplier = w.max_multiplier;
         info->block_size = w.block_size;
         info->autotuned = w.tuned;
+        info->factor_slices_resident = w.factors_resident;
+        info->factor_slices_rebuilt = w.factors_rebuilt;
         info->seconds_setup = w.t_setup;
         info->seconds_slices = w.t_slices;
         info->seconds_refine = w.t_refine;

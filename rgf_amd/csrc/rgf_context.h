@@ -89,6 +89,8 @@ struct Work {
     double max_multiplier = 0;  // largest slice-inverse multiplier
     int block_size = 0;         // slice-inverse block width used
     int tuned = 0;              // 1: this solve ran the block-width tuner
+    int64_t factors_resident = 0;  // mixed path: slice inverses kept resident
+    int64_t factors_rebuilt = 0;   // mixed path: slice inverses rebuilt by the sweeps
     double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
 };
 
