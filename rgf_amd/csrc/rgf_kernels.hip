@@ -1014,11 +1014,13 @@ static void launch_cgemm(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T>
 
 // K chunks for a split-K product with `tiles` 64x64 output tiles: about
 // 1536 work groups in one launch (measured better than one or two full
-// rounds of resident groups), chunks of at least 256; 0 = no split.
+// rounds of resident groups), chunks of at least 256 for K >= 1024 and of
+// at least 64 below (thin products of small slices: without the split their
+// few tiles run on a few CUs); 0 = no split.
 template <typename T>
 static int64_t splitk_chunks(int64_t tiles, int64_t K, size_t ws_elems, int64_t per, int64_t& kc_out) {
     constexpr int BK = GemmCfg<T>::BK_SMALL;
-    int64_t nz = std::min<int64_t>((1536 + tiles - 1) / tiles, K / 256);
+    int64_t nz = std::min<int64_t>((1536 + tiles - 1) / tiles, K / (K >= 1024 ? 256 : 64));
     nz = std::min<int64_t>(nz, (int64_t)(ws_elems / per));
     if (nz < 2) return 0;
     kc_out = ((K + nz - 1) / nz + BK - 1) / BK * BK;
@@ -1056,7 +1058,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
     if (M <= 0 || N <= 0) return;
     const bool any_skip = sk.rhi > sk.rlo || sk.chi > sk.clo || sk.bhi > sk.blo;
     // split-K for thin products with a long inner dimension
-    if (ws && !any_skip && K >= 1024) {
+    if (ws && !any_skip && K >= 256) {
         const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
         if (tiles < 512) {
             const int64_t per = 2 * M * N;  // elements of one partial (two planes)
@@ -1177,7 +1179,7 @@ void sweep_forward(hipStream_t s, int64_t N, int64_t m, CMat<float> G, CMat<floa
     const int64_t tiles = ((N + 63) / 64) * ((m + 63) / 64);
     const int64_t per = 2 * N * m;
     int64_t nz = 0, kc = 0;
-    if (ws && N >= 1024 && tiles < 512) nz = splitk_chunks<float>(tiles, N, ws_elems, per, kc);
+    if (ws && N >= 256 && tiles < 512) nz = splitk_chunks<float>(tiles, N, ws_elems, per, kc);
     if (nz >= 2) {
         GemmArgs<float> g = gemm_args<float>(N, m, N, G, t, CMat<float>(ws, ws + N * m, m), cd(1, 0), 0, Skip{});
         g.nz = (int)nz;
