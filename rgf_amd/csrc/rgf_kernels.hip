@@ -1393,17 +1393,34 @@ __device__ __forceinline__ void gj_diag_blk(T* Are, T* Aim, int64_t lda, int p0,
     // partial pivoting.
     {
         const int g = lane >> 4, c = lane & 15;
-        // the block (identity padding outside nb) through LDS: coalesced
-        // global loads, then each wavefront takes its block column
-        for (int e = tid; e < NB * NB; e += NT) {
-            const int i = e / NB, j = e % NB;
-            T vr = (i == j) ? T(1) : T(0), vi = 0;
-            if (i < nb && j < nb) {
-                vr = Are[(int64_t)(p0 + i) * lda + p0 + j];
-                vi = Aim[(int64_t)(p0 + i) * lda + p0 + j];
+        const int jc = wave * 16 + c;
+        // the block (identity padding outside nb) through LDS: coalesced rows
+        // (thread t: column t % NB of rows t / NB + 4k), eight loads in flight
+        // per thread before their LDS stores, then each wavefront takes its
+        // block column
+        {
+            static_assert(NT % NB == 0 && (NB * NB / NT) % 8 == 0, "bad block load tiling");
+            constexpr int RS = NT / NB, PER = NB * NB / NT;
+            const int j = tid % NB, i0 = tid / NB;
+#pragma unroll
+            for (int k0 = 0; k0 < PER; k0 += 8) {
+                T vr[8], vi[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int i = i0 + (k0 + u) * RS;
+                    vr[u] = (i == j) ? T(1) : T(0);
+                    vi[u] = 0;
+                    if (i < nb && j < nb) {
+                        vr[u] = Are[(int64_t)(p0 + i) * lda + p0 + j];
+                        vi[u] = Aim[(int64_t)(p0 + i) * lda + p0 + j];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    sr[i0 + (k0 + u) * RS][j] = vr[u];
+                    si[i0 + (k0 + u) * RS][j] = vi[u];
+                }
             }
-            sr[i][j] = vr;
-            si[i][j] = vi;
         }
         __syncthreads();
         V xr[NW], xi[NW];
@@ -1411,8 +1428,8 @@ __device__ __forceinline__ void gj_diag_blk(T* Are, T* Aim, int64_t lda, int p0,
         for (int bi = 0; bi < NW; ++bi)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                xr[bi][r] = sr[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c];
-                xi[bi][r] = si[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c];
+                xr[bi][r] = sr[bi * 16 + Mfma<T>::row(g, r)][jc];
+                xi[bi][r] = si[bi * 16 + Mfma<T>::row(g, r)][jc];
             }
         __syncthreads();
         // LDS views (aliasing sr / si): column panel [2][NB][CLD], D [2][16][CLD]
@@ -1601,23 +1618,19 @@ __device__ __forceinline__ void gj_diag_blk(T* Are, T* Aim, int64_t lda, int p0,
             ok = ok && okp[p] && isfinite(gr) && gr <= lim;
         }
         if (ok) {
+            // D^-1 from the registers to A and to Dout
 #pragma unroll
             for (int bi = 0; bi < NW; ++bi)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    sr[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c] = xr[bi][r];
-                    si[bi * 16 + Mfma<T>::row(g, r)][wave * 16 + c] = xi[bi][r];
+                    const int i = bi * 16 + Mfma<T>::row(g, r);
+                    if (i < nb && jc < nb) {
+                        Are[(int64_t)(p0 + i) * lda + p0 + jc] = xr[bi][r];
+                        Aim[(int64_t)(p0 + i) * lda + p0 + jc] = xi[bi][r];
+                        Dre[(int64_t)i * ldd + jc] = xr[bi][r];
+                        Dim[(int64_t)i * ldd + jc] = xi[bi][r];
+                    }
                 }
-            __syncthreads();
-            for (int e = tid; e < NB * NB; e += NT) {
-                const int i = e / NB, j = e % NB;
-                if (i < nb && j < nb) {
-                    Are[(int64_t)(p0 + i) * lda + p0 + j] = sr[i][j];
-                    Aim[(int64_t)(p0 + i) * lda + p0 + j] = si[i][j];
-                    Dre[(int64_t)i * ldd + j] = sr[i][j];
-                    Dim[(int64_t)i * ldd + j] = si[i][j];
-                }
-            }
             return;
         }
     }
