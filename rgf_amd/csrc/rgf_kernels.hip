@@ -1269,17 +1269,23 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
-// Value of lane 16 * (lane / 16) + k (the same lane group, column k) via
-// ds_swizzle in bit mode: source = ((lane & 0x10) | k) within each half wave.
+// Value of lane 16 * (lane / 16) + k (the same lane group, column k): the
+// DPP row broadcast (row_newbcast:k, a VALU move) instead of a permute
+// through the LDS unit, whose latency sits on the serial chain of the
+// pivot-tile elimination.
+template <int K>
+__device__ __forceinline__ int bcast16_bits(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+}
 template <int K>
 __device__ __forceinline__ float bcast16(float v) {
-    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x10 | (K << 5)));
+    return __int_as_float(bcast16_bits<K>(__float_as_int(v)));
 }
 template <int K>
 __device__ __forceinline__ double bcast16(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffLL), 0x10 | (K << 5));
-    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x10 | (K << 5));
+    const int lo = bcast16_bits<K>((int)(b & 0xffffffffLL));
+    const int hi = bcast16_bits<K>((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -1300,7 +1306,7 @@ __device__ __forceinline__ void cinv(double ar, double ai, double& ivr, double& 
 // complex tile held in the MFMA accumulator layout (lane (g, c) = (l/16, l%16)
 // holds column c, rows Mfma<T>::row(g, r)), without row interchanges.  The
 // pivot row comes by ds_bpermute from lane group grp(k), the multipliers
-// M[i][k] by ds_swizzle from lane c = k of the own group, the pivot by
+// M[i][k] by a DPP row broadcast from lane c = k of the own group, the pivot by
 // readlane.  bad is set on a zero or non-finite pivot; gm tracks the largest
 // |x|^2 of the new values (the growth of the unpivoted elimination).
 template <typename T, int K>
