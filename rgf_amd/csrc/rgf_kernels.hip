@@ -524,6 +524,10 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
     constexpr int TM = BM / WM, TN = BN / WN;
     constexpr int FM = TM / 16, FN = TN / 16;
     constexpr int LA = BK + 4, LB = BN + 4;  // padded LDS rows (16-byte multiples)
+    // K blocks of KB = 16 (MFMA steps s = 0..3 at lane group g take k = 4g + s)
+    // or, for BK = 8, of KB = 8 (two steps, k = 2g + s)
+    constexpr int KB = BK >= 16 ? 16 : 8, KS = KB / 4;
+    static_assert(BK % KB == 0 && KS % VW == 0, "K chunk must be 8 or a multiple of 16");
     __shared__ __align__(16) T sA[2][2][BM][LA];
     __shared__ __align__(16) T sB[2][2][BK][LB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -672,15 +676,15 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
     // MFMA work of the K chunk in LDS buffer buf
     auto chunk = [&](int buf) {
 #pragma unroll
-        for (int kb = 0; kb < BK; kb += 16) {
-            T ar[FM][4], ai[FM][4], br[FN][4], bi[FN][4];
+        for (int kb = 0; kb < BK; kb += KB) {
+            T ar[FM][KS], ai[FM][KS], br[FN][KS], bi[FN][KS];
 #pragma unroll
             for (int a = 0; a < FM; ++a) {
                 const int row = wm * TM + a * 16 + l16;
 #pragma unroll
-                for (int u = 0; u < 4; u += VW) {
-                    const VT xr = *(const VT*)&sA[buf][0][row][kb + 4 * g4 + u];
-                    const VT xi = *(const VT*)&sA[buf][1][row][kb + 4 * g4 + u];
+                for (int u = 0; u < KS; u += VW) {
+                    const VT xr = *(const VT*)&sA[buf][0][row][kb + KS * g4 + u];
+                    const VT xi = *(const VT*)&sA[buf][1][row][kb + KS * g4 + u];
 #pragma unroll
                     for (int v = 0; v < VW; ++v) {
                         ar[a][u + v] = xr[v];
@@ -692,16 +696,16 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
             for (int b = 0; b < FN; ++b) {
                 const int col = wn * TN + b * 16 + bperm(l16);
 #pragma unroll
-                for (int s2 = 0; s2 < 4; ++s2) {
-                    br[b][s2] = sB[buf][0][kb + 4 * g4 + s2][col];
-                    bi[b][s2] = sB[buf][1][kb + 4 * g4 + s2][col];
+                for (int s2 = 0; s2 < KS; ++s2) {
+                    br[b][s2] = sB[buf][0][kb + KS * g4 + s2][col];
+                    bi[b][s2] = sB[buf][1][kb + KS * g4 + s2][col];
                 }
             }
             // x * y accumulated into the tile of C (TR: of C^T)
             auto mm = [](T x, T y, V c) { return TR ? Mfma<T>::mma(y, x, c) : Mfma<T>::mma(x, y, c); };
             if constexpr (M3) {
 #pragma unroll
-                for (int s2 = 0; s2 < 4; ++s2) {
+                for (int s2 = 0; s2 < KS; ++s2) {
 #pragma unroll
                     for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -722,7 +726,7 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
                 }
             } else
 #pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2) {
+            for (int s2 = 0; s2 < KS; ++s2) {
 #pragma unroll
                 for (int a = 0; a < FM; ++a)
 #pragma unroll
@@ -871,14 +875,15 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
 }
 
 // FP32 64x64 tiles: four work groups per CU (four waves per SIMD, at most
-// 128 registers); the three-product accumulators need all of them.
-template <typename T, int WM, int WN>
+// 128 registers); the three-product accumulators need all of them.  FP64
+// 64x64 tiles with 8-deep K chunks: three (at most 168 registers).
+template <typename T, int WM, int WN, int BK>
 constexpr int gemm_min_blocks() {
-    return (sizeof(T) == 4 && WM * WN == 4) ? 4 : 1;
+    return (sizeof(T) == 4 && WM * WN == 4) ? 4 : (WM * WN == 4 && BK == 8) ? 3 : 1;
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool ACC3, int MAP, bool TR, bool M3, bool AMAX = false>
-__global__ __launch_bounds__(WM* WN * 64, (gemm_min_blocks<T, WM, WN>())) void cgemm2_kernel(
+__global__ __launch_bounds__(WM* WN * 64, (gemm_min_blocks<T, WM, WN, BK>())) void cgemm2_kernel(
     GemmArgs<T> g0, GemmArgs<T> g1, GemmArgs<T> g2, int total) {
     int bid = (int)blockIdx.x;
     if constexpr ((MAP & 1) != 0) {
@@ -895,13 +900,19 @@ template <typename T>
 struct GemmCfg;
 // FP64: 64x64 tiles, 4 waves (32x32 per wave); 128x128, 8 waves (64x32).
 // FP32: same shapes with a deeper K chunk.
+// BK2: K chunk of the streamlined 64x64 kernel.  FP64 takes 8: 42 KB of LDS
+// and at most 168 registers, three work groups per CU instead of two (the
+// rank-64 trailing update at N = 4096: 172 instead of 198 us alone, the FP64
+// slice inverse 13.0 instead of 14.8 ms, with the panel and next-block
+// launches on the same footprint so that they take the places the update's
+// groups free).
 template <>
 struct GemmCfg<double> {
-    static constexpr int BK_SMALL = 16, BK_BIG = 8;
+    static constexpr int BK_SMALL = 16, BK_BIG = 8, BK2 = 8;
 };
 template <>
 struct GemmCfg<float> {
-    static constexpr int BK_SMALL = 32, BK_BIG = 32;
+    static constexpr int BK_SMALL = 32, BK_BIG = 32, BK2 = 16;
 };
 
 template <typename T>
@@ -989,15 +1000,18 @@ static bool launch_pair2(hipStream_t s, const GemmArgs<T>& g0_, const GemmArgs<T
     auto tiles = [](GemmArgs<T>& g) {
         int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
         if constexpr ((MAP & 2) != 0) {
-            // leave whole skipped tile rows and columns out of the grid
-            if (g.srhi > g.srlo && g.srlo % BM == 0 && g.srhi % BM == 0 && g.srhi <= g.M) {
+            // leave whole skipped tile rows and columns out of the grid (a
+            // skip that reaches past the last row or column covers the tiles
+            // up to the end of the grid)
+            const int rhi = g.srhi >= g.M ? tm * BM : g.srhi, chi = g.schi >= g.N ? tn * BN : g.schi;
+            if (rhi > g.srlo && g.srlo % BM == 0 && rhi % BM == 0) {
                 g.cr_lo = g.srlo / BM;
-                g.cr_n = (g.srhi - g.srlo) / BM;
+                g.cr_n = (rhi - g.srlo) / BM;
                 tm -= g.cr_n;
             }
-            if (g.schi > g.sclo && g.sclo % BN == 0 && g.schi % BN == 0 && g.schi <= g.N) {
+            if (chi > g.sclo && g.sclo % BN == 0 && chi % BN == 0) {
                 g.cc_lo = g.sclo / BN;
-                g.cc_n = (g.schi - g.sclo) / BN;
+                g.cc_n = (chi - g.sclo) / BN;
                 tn -= g.cc_n;
             }
         }
@@ -1100,7 +1114,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
                 g.nz = (int)nz;
                 g.kc = (int)kc;
                 g.zstride = per;
-                if (!launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g))
+                if (!launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g))
                     launch_pair<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, g, nullptr, use3m);
                 hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
                                    (const T*)ws, per, C.re, C.im, C.ld, alpha.real(), alpha.imag(), beta);
@@ -1120,7 +1134,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
         // streamlined kernel: 64x64 tiles (4 waves, K chunk 16) when there
         // are enough tiles, else 32x32
         const GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, C, alpha, beta, sk);
-        const bool done = tiles_mid >= 256 ? launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g)
+        const bool done = tiles_mid >= 256 ? launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g)
                                            : launch_cplx<T, 32, 32, 16, 1, 1>(use3m, s, g);
         if (done) {
             RGF_HIP(hipGetLastError());
@@ -1143,7 +1157,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
 template <typename T>
 static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, int use3m,
                        hipEvent_t stop = nullptr) {
-    if (!launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g0, &g1, nullptr, stop)) {
+    if (!launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g0, &g1, nullptr, stop)) {
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
         if (stop) RGF_HIP(hipEventRecord(stop, s));
     }
@@ -1154,7 +1168,7 @@ static void cgemm_pair(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& 
 template <typename T>
 static void cgemm_triple(hipStream_t s, const GemmArgs<T>& g0, const GemmArgs<T>& g1, const GemmArgs<T>& g2,
                          int use3m, hipEvent_t stop = nullptr) {
-    if (!launch_cplx<T, 64, 64, 16, 2, 2>(use3m, s, g0, &g1, &g2, stop)) {
+    if (!launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g0, &g1, &g2, stop)) {
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g0, &g1, use3m);
         launch_pair<T, 32, 32, GemmCfg<T>::BK_SMALL, 1, 1>(s, g2, nullptr, use3m);
         if (stop) RGF_HIP(hipEventRecord(stop, s));
@@ -2161,8 +2175,9 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
             // the block edges are on the 64-wide tile grid
             const GemmArgs<T> g =
                 gemm_args<T>(N, N, nb, Aold[q & 1], A.offset(p0, 0), A, cd(-1, 0), 1, Skip{p0, p2, p0, p2, p2, p3});
-            const bool done = use3m ? launch_pair2<T, 64, 64, 16, 2, 2, false, 2, true, true>(s2, g, nullptr)
-                                    : launch_pair2<T, 64, 64, 16, 2, 2, false, 2, true, false>(s2, g, nullptr);
+            constexpr int BK = GemmCfg<T>::BK2;
+            const bool done = use3m ? launch_pair2<T, 64, 64, BK, 2, 2, false, 2, true, true>(s2, g, nullptr)
+                                    : launch_pair2<T, 64, 64, BK, 2, 2, false, 2, true, false>(s2, g, nullptr);
             if (!done)
                 cgemm_skip<T>(s2, N, N, nb, Aold[q & 1], A.offset(p0, 0), A, cd(-1, 0), 1,
                               Skip{p0, p2, p0, p2, p2, p3}, use3m, nullptr, 0);
