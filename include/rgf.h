@@ -286,11 +286,14 @@ rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double*
  *     double for RGF_PRECISION_FP64, float for RGF_PRECISION_FP32_REFINED
  *     (| RGF_DENSE_3M as in rgf_dense_inverse).
  *   status: device memory for two int32 words, or NULL.  Given, the call is
- *     asynchronous on `stream` and leaves [0] != 0 on a zero or non-finite
- *     pivot and in [1] the largest multiplier squared (float bits, see
+ *     asynchronous on `stream` and leaves bit 1 of [0] set on a zero or
+ *     non-finite pivot, bit 2 if a hand-over between the work groups of one
+ *     launch timed out (an internal error: the result is not valid), and in
+ *     [1] the largest multiplier squared (float bits, see
  *     rgf_info.max_multiplier); NULL: the call waits for the inverse and
  *     returns RGF_ERR_NUMERICAL on a zero pivot (or, in FP64, a multiplier
- *     above 1e6), as rgf_dense_inverse does.
+ *     above 1e6), RGF_ERR_DEVICE on a hand-over time-out, as
+ *     rgf_dense_inverse does.
  *   stream: a HIP stream (hipStream_t), or NULL for the handle's stream.
  * n beyond the workspace (about 8160 at the default block sizes) is refused
  * with RGF_ERR_INVALID_ARGUMENT.  Every call on one GPU shares that GPU's

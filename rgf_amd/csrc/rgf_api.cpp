@@ -117,6 +117,14 @@ std::vector<cd> download(rgf_context* c, CMat<double> d, int64_t rows, int64_t c
 // amplify the rounding of every later update by about that factor.
 double multiplier_limit(bool fp64) { return fp64 ? 1e6 : 1e3; }
 
+// The flags word of the slice-inverse status: bit 1 a zero or non-finite
+// pivot, bit 2 a hand-over between the work groups of a panel launch that
+// timed out (the block-two-ahead step then read unfinished panel tiles).
+void status_error(int flags) {
+    if (flags & 2) throw Error(RGF_ERR_DEVICE, "a work-group hand-over in a slice inverse timed out");
+    if (flags) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+}
+
 // Status of the slice inverses since the last reset_status: a zero or
 // non-finite pivot is an error; so is a multiplier above multiplier_limit
 // when `growth` (the FP64 path and the dense inverse; the mixed path's FP64
@@ -125,7 +133,7 @@ double check_status(rgf_context* c, bool growth = true, bool fp64 = true) {
     int st[2] = {0, 0};
     RGF_HIP(hipMemcpyAsync(st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
     RGF_HIP(hipStreamSynchronize(c->stream));
-    if (st[0]) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+    status_error(st[0]);
     float m2;
     memcpy(&m2, &st[1], sizeof m2);
     const double m = std::sqrt((double)m2);
@@ -1630,7 +1638,7 @@ rgf_status rgf_dense_inverse_noalloc(rgf_handle h, int32_t precision, int64_t n,
             int w[2] = {0, 0};
             RGF_HIP(hipMemcpyAsync(w, st, sizeof w, hipMemcpyDeviceToHost, s));
             RGF_HIP(hipStreamSynchronize(s));
-            if (w[0]) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+            status_error(w[0]);
             float m2;
             memcpy(&m2, &w[1], sizeof m2);
             if (fp64 && !(std::sqrt((double)m2) <= multiplier_limit(true)))

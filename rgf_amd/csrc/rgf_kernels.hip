@@ -129,7 +129,22 @@ struct GemmArgs {
     // optional: max |C_ij|^2 over the stored entries, as the bits of a
     // non-negative float (which order like unsigned integers), atomicMax'ed
     unsigned* amax;
+    // optional hand-over between the problems of one launch (work groups are
+    // dispatched in grid order, so a waiting problem must come after the
+    // problem it waits for): a tile whose columns meet [ctr_clo, ctr_chi)
+    // adds 1 to *ctr after its stores are visible device-wide; a problem with
+    // ctr_wait > 0 waits until *ctr reaches it before reading its operands
+    // (bounded: on a time-out bit 2 of *fail is set and the tile goes on)
+    unsigned* ctr;
+    int ctr_clo, ctr_chi;
+    unsigned ctr_wait;
+    int* fail;
 };
+
+// Spin limit of a hand-over wait (2^20 polls of the counter, each a
+// device-scope load and a 64-cycle sleep: at most about a second; the tiles
+// waited for take tens of microseconds).
+constexpr unsigned kHandoverSpins = 1u << 20;
 
 // max |x|^2 of a wavefront's values into *amax (non-negative float bits)
 __device__ __forceinline__ void wave_amax(unsigned* amax, float m) {
@@ -516,6 +531,33 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
     if (tile_m >= g.srlo && tile_m + BM <= g.srhi) return;
     if (tile_n >= g.sclo && tile_n + BN <= g.schi) return;
     if (tile_m >= g.sblo && tile_m + BM <= g.sbhi && tile_n >= g.sblo && tile_n + BN <= g.sbhi) return;
+    if (g.ctr_wait) {
+        if (threadIdx.x == 0) {
+            unsigned it = 0;
+            while (__hip_atomic_load(g.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g.ctr_wait) {
+                if (++it >= kHandoverSpins) {
+                    atomicOr(g.fail, 2);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    // after the stores of a tile that another problem of the launch waits for
+    unsigned* const hctr = g.ctr;
+    const bool hcount = hctr && tile_n < g.ctr_chi && tile_n + BN > g.ctr_clo;
+    auto hand_over = [hctr, hcount]() {
+        if (hcount) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(hctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    };
+    // the tiles of a hand-over are the critical path of their launch: their
+    // waves take issue priority over the trailing update's on shared SIMDs
+    if (hcount || g.ctr_wait) __builtin_amdgcn_s_setprio(3);
 
     using V = typename Mfma<T>::v4;
     using VT = typename Vec<T>::type;
@@ -845,6 +887,7 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
                 }
             }
         if (AMAX && amax) wave_amax(amax, mloc);
+        hand_over();
         return;
     }
 #pragma unroll
@@ -872,6 +915,7 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
             }
         }
     if (AMAX && amax) wave_amax(amax, mloc);
+    hand_over();
 }
 
 // FP32 64x64 tiles: four work groups per CU (four waves per SIMD, at most
@@ -957,6 +1001,10 @@ static GemmArgs<T> gemm_args(int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<T>
     g.vec_ok = (A.ld % VW == 0) && (B.ld % VW == 0) && ((uintptr_t)A.re % 16 == 0) && ((uintptr_t)A.im % 16 == 0) &&
                ((uintptr_t)B.re % 16 == 0) && ((uintptr_t)B.im % 16 == 0);
     g.amax = nullptr;
+    g.ctr = nullptr;
+    g.ctr_clo = g.ctr_chi = 0;
+    g.ctr_wait = 0;
+    g.fail = nullptr;
     return g;
 }
 
@@ -1913,11 +1961,12 @@ __global__ void gj_writeback2_kernel(T* Are, T* Aim, int64_t lda, int64_t N, int
 }
 
 // Copies of the block column A[:, p0:p0+nb] (to C, ld ldc) and block row
-// A[p0:p0+nb, :] (to R, ld ldr), both planes.
+// A[p0:p0+nb, :] (to R, ld ldr), both planes; *zero (if given) is cleared.
 template <typename T>
 __global__ void panel_copy_kernel(const T* Are, const T* Aim, int64_t lda, int64_t N, int64_t p0, int nb, T* Cre,
-                                  T* Cim, int64_t ldc, T* Rre, T* Rim, int64_t ldr) {
+                                  T* Cim, int64_t ldc, T* Rre, T* Rim, int64_t ldr, unsigned* zero) {
     const int64_t half = (int64_t)nb * N;
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < 2 * half;
          e += (int64_t)gridDim.x * blockDim.x) {
         if (e < half) {
@@ -2094,6 +2143,29 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
     // blocks the next copies need from these panels, A[Q, P] and A[P, Q], are
     // also stored to Rold[:, P] and Aold[Q][P, :] (Rold[:, P] is not read
     // here: the tiles of columns P are skipped).
+    // A[K, K] -= Aold_j[K, :] A[Pj, K]: step j applied to diagonal block k;
+    // `stop` is recorded when it completes
+    auto schur = [&](int64_t k, int64_t j, hipEvent_t stop) {
+        const int64_t pk = p_of(k), pj = p_of(j);
+        const int nk = nb_of(k), nj = nb_of(j);
+        const GemmArgs<T> g = gemm_args<T>(nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk),
+                                           cd(-1, 0), 1, Skip{});
+        if (!launch_cplx<T, 32, 32, 16, 1, 1>(use3m, s, g, nullptr, nullptr, stop)) {
+            cgemm_skip<T>(s, nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk), cd(-1, 0), 1,
+                          Skip{}, use3m, nullptr, 0);
+            if (stop) RGF_HIP(hipEventRecord(stop, s));
+        }
+    };
+    // FP32: the step on the block two ahead rides in the panel launch (one
+    // launch and one launch gap fewer on the diagonal chain, which sets the
+    // FP32 step time: 1.4-2.6 % off the inverse at N = 4096).  Its tiles come
+    // last in the grid and wait for the row-panel tiles they read (the
+    // columns of block q+1), which count themselves into a counter kept
+    // after the panel copies; `handed` is the count the device reaches once
+    // every launched hand-over tile is done.  FP64 keeps the separate launch
+    // (its step time is the bulk update's; fused measured 0.3 % slower).
+    unsigned* const hand = reinterpret_cast<unsigned*>(ws + 2 * ((size_t)3 * b * N + (size_t)b * b));
+    unsigned handed = 0;
     auto panels_inplace = [&](int64_t q, hipEvent_t stop) {
         const int64_t p0 = p_of(q), p1 = p0 + nb_of(q);
         const int nb = nb_of(q), nb1 = nb_of(q + 1);
@@ -2117,20 +2189,36 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
             gc.y_clo = 0;
             gc.y_chi = nb;
         }
-        cgemm_pair<T>(s, gr, gc, use3m, stop);
-    };
-    // A[K, K] -= Aold_j[K, :] A[Pj, K]: step j applied to diagonal block k;
-    // `stop` is recorded when it completes
-    auto schur = [&](int64_t k, int64_t j, hipEvent_t stop) {
-        const int64_t pk = p_of(k), pj = p_of(j);
-        const int nk = nb_of(k), nj = nb_of(j);
-        const GemmArgs<T> g = gemm_args<T>(nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk),
-                                           cd(-1, 0), 1, Skip{});
-        if (!launch_cplx<T, 32, 32, 16, 1, 1>(use3m, s, g, nullptr, nullptr, stop)) {
-            cgemm_skip<T>(s, nk, nk, nj, Aold[j & 1].offset(pk, 0), A.offset(pj, pk), A.offset(pk, pk), cd(-1, 0), 1,
-                          Skip{}, use3m, nullptr, 0);
-            if (stop) RGF_HIP(hipEventRecord(stop, s));
+        if (nb1 > 0 && sizeof(T) == 8) {
+            cgemm_pair<T>(s, gr, gc, use3m);
+            schur(q + 1, q, stop);
+            return;
         }
+        if (nb1 > 0) {
+            // step q on block q+1: A[Q, Q] -= Aold_q[Q, :] A[P, Q], A[P, Q] from this launch's row panel
+            constexpr int BK = GemmCfg<T>::BK2;
+            GemmArgs<T> gs = gemm_args<T>(nb1, nb1, nb, Aold[q & 1].offset(p1, 0), A.offset(p0, p1), A.offset(p1, p1),
+                                          cd(-1, 0), 1, Skip{});
+            unsigned count = 0;  // row-panel tiles (64 x 64) meeting the columns of block q+1
+            for (int64_t t = 0; t < N; t += 64)
+                if (t < p1 + nb1 && t + 64 > p1 && !(t >= p0 && t + 64 <= p1)) count += (unsigned)((nb + 63) / 64);
+            GemmArgs<T> gh = gr;
+            gh.ctr = hand;
+            gh.ctr_clo = (int)p1;
+            gh.ctr_chi = (int)(p1 + nb1);
+            gs.ctr = hand;
+            gs.ctr_wait = handed + count;
+            gs.fail = dev_status;
+            if (launch_cplx<T, 64, 64, BK, 2, 2>(use3m, s, gh, &gc, &gs, stop)) {
+                handed += count;
+            } else {
+                cgemm_pair<T>(s, gr, gc, use3m);
+                schur(q + 1, q, stop);
+            }
+            RGF_HIP(hipGetLastError());
+            return;
+        }
+        cgemm_pair<T>(s, gr, gc, use3m, stop);
     };
     {
         // step 0 panels from full copies of block row and column 0
@@ -2138,9 +2226,8 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         const int64_t tot = 2 * (int64_t)nb * N;
         launch_diag<T>(s, A, 0, nb, D, dev_status);
         hipLaunchKernelGGL((panel_copy_kernel<T>), dim3(grid_for(tot, 256, 4096)), dim3(256), 0, s, A.re, A.im, A.ld,
-                           N, (int64_t)0, nb, Aold[0].re, Aold[0].im, Aold[0].ld, Rold.re, Rold.im, Rold.ld);
-        panels_inplace(0, nullptr);
-        schur(1, 0, evS[0]);
+                           N, (int64_t)0, nb, Aold[0].re, Aold[0].im, Aold[0].ld, Rold.re, Rold.im, Rold.ld, hand);
+        panels_inplace(0, evS[0]);  // with step 0 on block 1
         launch_diag<T>(s, A, p_of(1), nb_of(1), D, dev_status);
     }
     for (int64_t q = 0; q < nbl; ++q) {
@@ -2189,13 +2276,9 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
         RGF_HIP(hipEventRecord(evB[q & 1], s2));
         if (nb1 > 0) {
             RGF_HIP(hipStreamWaitEvent(s, evT[q & 1], 0));
-            if (nb2 > 0) {
-                panels_inplace(q + 1, nullptr);
-                schur(q + 2, q + 1, evS[(q + 1) & 1]);
-                launch_diag<T>(s, A, p2, nb2, D, dev_status);
-            } else {
-                panels_inplace(q + 1, evS[(q + 1) & 1]);
-            }
+            // Q's panels and step q+1 on (Q2, Q2) in one launch, then Q2's inverse
+            panels_inplace(q + 1, evS[(q + 1) & 1]);
+            if (nb2 > 0) launch_diag<T>(s, A, p2, nb2, D, dev_status);
         }
     }
     RGF_HIP(hipEventRecord(ov->ev[4], s2));
@@ -2824,7 +2907,13 @@ __device__ __forceinline__ GemmArgs<T> dev_args(int M, int N, int K, const T* Ar
     g.zstride = 0;
     g.tiles_n = tiles_n;
     g.tiles = 0;
+    g.cr_lo = g.cc_lo = INT32_MAX;
+    g.cr_n = g.cc_n = 0;
     g.amax = nullptr;
+    g.ctr = nullptr;
+    g.ctr_clo = g.ctr_chi = 0;
+    g.ctr_wait = 0;
+    g.fail = nullptr;
     return g;
 }
 

@@ -128,6 +128,9 @@ def lookahead_schedule(N, b):
           work group that needs a whole CU, is queued first);
       s : Q's panels in place from the copies (D_Q from the inverse queued in
           the previous pass), step q+1 on (Q2, Q2), the inverse of Q2 into D.
+    In FP32 the step on (Q2, Q2) is part of the panel launch (its tiles wait
+    inside the launch for the row-panel tiles A[Q, Q2] they read); the model
+    keeps it as the next launch on s, the order that hand-over enforces.
     """
     m = Model()
     bl = blocks(N, b)
