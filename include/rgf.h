@@ -31,10 +31,11 @@
 extern "C" {
 #endif
 
-#define RGF_ABI_VERSION 4  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate;
+#define RGF_ABI_VERSION 5  /* 2: rgf_options.refine_s_tol, rgf_info.refine_s_err_estimate;
                               3: rgf_info.max_multiplier;
                               4: rgf_options.autotune and factor_slots, rgf_info.block_size,
-                                 autotuned, factor_slices_resident and factor_slices_rebuilt */
+                                 autotuned, factor_slices_resident and factor_slices_rebuilt;
+                              5: rgf_options.pivoting, rgf_info.pivoted */
 #define RGF_ALL (-1) /* selection count meaning "all propagating channels 1..N_prop" */
 
 typedef enum {
@@ -130,6 +131,15 @@ typedef struct {
                                  rebuilt in segments by each refinement sweep from
                                  stored checkpoints (memory O((R + L/C + C) N^2)
                                  instead of O(L N^2), more inverses computed)     */
+    int32_t pivoting;         /* slice inverses: 0 (default) = blocked Gauss-Jordan with
+                                 pivoting inside each diagonal block; a slice it
+                                 refuses (zero pivot, or multipliers above 1e6 in
+                                 FP64; in FP32+refine a refinement that stalls after
+                                 multipliers above 1e3) has the solve run again with
+                                 every slice inverted with partial pivoting over all
+                                 rows (LAPACK getrf + getri, as MATLAB's inv; much
+                                 slower); 1 = that from the start; 2 = no fallback
+                                 (the refusal is returned)                         */
 } rgf_options;
 
 typedef struct {
@@ -166,6 +176,8 @@ typedef struct {
        how many the sweeps rebuilt */
     int64_t factor_slices_resident;
     int64_t factor_slices_rebuilt;
+    int32_t pivoted;          /* 1: the slice inverses used partial pivoting over all
+                                 rows (rgf_options.pivoting)                          */
 } rgf_info;
 
 /* Lead channels (setup_channels.m:30-82; setup_channels_3D.m:30-64).
@@ -271,7 +283,9 @@ rgf_status rgf_transmission_gradient(rgf_handle h, const rgf_problem* p,
  * rgf_dense_inverse: in-place inverse of an n x n complex matrix
  * (interleaved, column-major) with the slice-inverse kernel of `precision`;
  * RGF_PRECISION_FP32_REFINED | RGF_DENSE_3M selects the FP32 kernels with
- * three real products per complex product (complex_3m bit 0).
+ * three real products per complex product (complex_3m bit 0).  A matrix the
+ * blocked kernels refuse is inverted again with partial pivoting over all
+ * rows (rgf_options.pivoting 0); block_size < 0 takes that path directly.
  * rgf_dense_gemm: C = A B, (m x k)(k x n), interleaved column-major. */
 #define RGF_DENSE_3M 0x100
 rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double* A, int32_t block_size);

@@ -54,7 +54,7 @@ class Options(C.Structure):
         ("block_size", C.c_int32), ("complex_3m", C.c_int32), ("profile", C.c_int32),
         ("single_stream", C.c_int32), ("batch_concurrency", C.c_int32), ("batch_engine", C.c_int32),
         ("batch_graph_rounds", C.c_int32), ("autotune", C.c_int32),
-        ("refine_s_tol", C.c_double), ("factor_slots", C.c_int32),
+        ("refine_s_tol", C.c_double), ("factor_slots", C.c_int32), ("pivoting", C.c_int32),
     ]
 
 
@@ -69,6 +69,7 @@ class Info(C.Structure):
         ("refine_s_err_estimate", C.c_double), ("max_multiplier", C.c_double),
         ("block_size", C.c_int32), ("autotuned", C.c_int32),
         ("factor_slices_resident", C.c_int64), ("factor_slices_rebuilt", C.c_int64),
+        ("pivoted", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

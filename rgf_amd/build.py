@@ -50,7 +50,11 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [cc, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
+    # the vendor dense solver (pivoted slice inverses, the fallback for slices
+    # the blocked kernels refuse)
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(cc)), "lib")
+    cmd = [cc, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-L" + rocm_lib, "-Wl,-rpath," + rocm_lib,
+           "-lrocsolver", "-lrocblas", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

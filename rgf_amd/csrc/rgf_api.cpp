@@ -21,6 +21,7 @@
 //    through the stored FP32 inverses and the residual uses the FP64 slice
 //    operators and FP64 lead Green's functions.
 #include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -117,12 +118,20 @@ std::vector<cd> download(rgf_context* c, CMat<double> d, int64_t rows, int64_t c
 // amplify the rounding of every later update by about that factor.
 double multiplier_limit(bool fp64) { return fp64 ? 1e6 : 1e3; }
 
+// A slice inverse the blocked Gauss-Jordan kernels refuse: a zero or
+// non-finite pivot, or multipliers above multiplier_limit (their pivoting is
+// confined to the diagonal block).  The solve drivers then invert every
+// slice again with partial pivoting over all rows (with_pivoting_fallback).
+struct SliceRefused : Error {
+    using Error::Error;
+};
+
 // The flags word of the slice-inverse status: bit 1 a zero or non-finite
 // pivot, bit 2 a hand-over between the work groups of a panel launch that
 // timed out (the block-two-ahead step then read unfinished panel tiles).
 void status_error(int flags) {
     if (flags & 2) throw Error(RGF_ERR_DEVICE, "a work-group hand-over in a slice inverse timed out");
-    if (flags) throw Error(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
+    if (flags) throw SliceRefused(RGF_ERR_NUMERICAL, "zero or non-finite pivot in a slice inverse");
 }
 
 // Status of the slice inverses since the last reset_status: a zero or
@@ -143,12 +152,84 @@ double check_status(rgf_context* c, bool growth = true, bool fp64 = true) {
                  "ill-conditioned diagonal block in a slice inverse: multiplier %.3g above %.0e (pivoting is "
                  "confined to the diagonal block, so the inverse would be inaccurate)",
                  m, multiplier_limit(fp64));
-        throw Error(RGF_ERR_NUMERICAL, buf);
+        throw SliceRefused(RGF_ERR_NUMERICAL, buf);
     }
     return m;
 }
 
 void reset_status(rgf_context* c) { RGF_HIP(hipMemsetAsync(c->d_status, 0, 2 * sizeof(int), c->stream)); }
+
+void rocblas_ok(rocblas_status st, const char* what) {
+    if (st != rocblas_status_success) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "%s failed (rocblas status %d)", what, (int)st);
+        throw Error(st == rocblas_status_memory_error ? RGF_ERR_OUT_OF_MEMORY : RGF_ERR_DEVICE, buf);
+    }
+}
+
+// The slice inverse with partial pivoting over all rows, as MATLAB's inv
+// (cal_smatrix_RGF.m:265; LAPACK getrf + getri), through the vendor's dense
+// solver library: the fallback for slices the blocked Gauss-Jordan kernels
+// refuse, and every slice with rgf_options.pivoting = 1.  The row-major
+// matrix is the column-major transpose and inv(A^T) = inv(A)^T, so its
+// interleaved copy goes to the solver as it is.  A zero pivot sets bit 1 of
+// the status word.
+template <typename T>
+void invert_pivoted(rgf_context* c, hipStream_t s, int64_t N, CMat<T> A) {
+    if (N > 46340) throw Error(RGF_ERR_INVALID_ARGUMENT, "matrix too large for the pivoted slice inverse");
+    if (!c->rb) {
+        rocblas_handle h = nullptr;
+        rocblas_ok(rocblas_create_handle(&h), "rocblas_create_handle");
+        c->rb = h;
+    }
+    rocblas_handle rb = (rocblas_handle)c->rb;
+    rocblas_ok(rocblas_set_stream(rb, s), "rocblas_set_stream");
+    T* M = dbuf<T>(c, "piv_m", (size_t)2 * N * N);
+    rocblas_int* ipiv = dbuf<rocblas_int>(c, "piv_ipiv", (size_t)N);
+    rocblas_int* info = dbuf<rocblas_int>(c, "piv_info", 2);
+    const rocblas_int n = (rocblas_int)N;
+    to_interleaved<T>(s, N, A, M);
+    if constexpr (sizeof(T) == 8) {
+        rocblas_ok(rocsolver_zgetrf(rb, n, n, (rocblas_double_complex*)M, n, ipiv, info), "rocsolver_zgetrf");
+        rocblas_ok(rocsolver_zgetri(rb, n, (rocblas_double_complex*)M, n, ipiv, info + 1), "rocsolver_zgetri");
+    } else {
+        rocblas_ok(rocsolver_cgetrf(rb, n, n, (rocblas_float_complex*)M, n, ipiv, info), "rocsolver_cgetrf");
+        rocblas_ok(rocsolver_cgetri(rb, n, (rocblas_float_complex*)M, n, ipiv, info + 1), "rocsolver_cgetri");
+    }
+    from_interleaved<T>(s, N, M, A);
+    fold_info(s, info, c->d_status);
+}
+
+// one slice inverse of a solve: the blocked kernels, or with w.pivot_all the
+// pivoted inverse
+template <typename T>
+void invert_slice(rgf_context* c, const Work& w, hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int use3m,
+                  KernelTimer* timer, const Overlap* ov) {
+    if (w.pivot_all)
+        invert_pivoted<T>(c, s, N, A);
+    else
+        invert_inplace<T>(s, N, A, b, ws, c->d_status, use3m, timer, ov);
+}
+
+// Runs a solve driver f on w.  rgf_options.pivoting 0 (default): when the
+// blocked slice inverses refuse a slice, or in the mixed precision when the
+// refinement stalls after multipliers above the FP32 limit, f runs again with
+// every slice inverted with partial pivoting over all rows; 1: pivoted from
+// the start; 2: no fallback (the refusal is the caller's error).
+void with_pivoting_fallback(const rgf_options& o, Work& w, const std::function<void()>& f) {
+    w.pivot_all = o.pivoting == 1;
+    try {
+        f();
+        return;
+    } catch (const Error& e) {
+        const bool refused = dynamic_cast<const SliceRefused*>(&e) != nullptr;
+        const bool stalled = e.status == RGF_ERR_NOT_CONVERGED && o.precision == RGF_PRECISION_FP32_REFINED &&
+                             !(w.max_multiplier <= multiplier_limit(false));
+        if (o.pivoting != 0 || !(refused || stalled)) throw;
+    }
+    w.pivot_all = 1;
+    f();
+}
 
 
 void upload_eps(rgf_context* c, const rgf_problem* p, Work& w) {
@@ -426,7 +507,7 @@ void run_carry(rgf_context* c, Work& w, const rgf_options& o) {
                                    cd(0, 0), w.k2, nullptr, w.A0);
         else
             assemble_slice<double>(s, N, G, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
-        invert_inplace<double>(s, N, G, b, ws, c->d_status, 0, w.timer, cfg.lookahead ? &c->ov : nullptr);
+        invert_slice<double>(c, w, s, N, G, b, ws, 0, w.timer, cfg.lookahead ? &c->ov : nullptr);
         if (pi) {
             cgemm<double>(s, N, pi, N, G, X, X2, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             std::swap(X, X2);
@@ -503,7 +584,7 @@ struct FactorStore {
             copy_mat<float>(s, N, N, prev, out);
             assemble_slice<float>(s, N, out, nullptr, nullptr, w->epsR, w->k2, &G0R32, w->A0);
         }
-        invert_inplace<float>(s, N, out, b, ws, c->d_status, use3m, timer, ov);
+        invert_slice<float>(c, *w, s, N, out, b, ws, use3m, timer, ov);
     }
     // segment k into the segment slots; in the first pass, also store the
     // checkpoint of segment k + 1
@@ -842,7 +923,7 @@ void run_gradient(rgf_context* c, Work& w, const rgf_options& o, int every, doub
             copy_mat<double>(s, N, N, prev, out);
             assemble_slice<double>(s, N, out, nullptr, nullptr, w.epsR, w.k2, &w.G0R, w.A0);
         }
-        invert_inplace<double>(s, N, out, b, ws, c->d_status, 0, w.timer, cfg.lookahead ? &c->ov : nullptr);
+        invert_slice<double>(c, w, s, N, out, b, ws, 0, w.timer, cfg.lookahead ? &c->ov : nullptr);
     };
     reset_status(c);
     RGF_HIP(hipStreamSynchronize(s));
@@ -1086,10 +1167,12 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
     w.t_setup = now() - t0;
     double T = 0;
     std::vector<double> gre, gim;
-    if (o.precision == RGF_PRECISION_FP64)
-        run_gradient(c, w, o, every, T, gre, gim);
-    else
-        run_gradient_refined(c, w, o, T, gre, gim);
+    with_pivoting_fallback(o, w, [&] {
+        if (o.precision == RGF_PRECISION_FP64)
+            run_gradient(c, w, o, every, T, gre, gim);
+        else
+            run_gradient_refined(c, w, o, T, gre, gim);
+    });
     if (T_out) *T_out = T;
     const int64_t N = w.N, L = w.L;
     for (int64_t n = 0; n < L; ++n) {
@@ -1108,6 +1191,7 @@ void gradient_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, 
         info->refine_residual = w.refine_residual;
         info->flops_slices = w.flops;
         info->max_multiplier = w.max_multiplier;
+        info->pivoted = w.pivot_all;
         info->block_size = w.block_size;
         info->autotuned = w.tuned;
         info->factor_slices_resident = w.factors_resident;
@@ -1205,10 +1289,12 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
     prepare(c, p, ch, sel, w);
     w.fields = fields;
     w.t_setup = now() - t0;
-    if (o.precision == RGF_PRECISION_FP64)
-        run_carry(c, w, o);
-    else
-        run_refined(c, w, o);
+    with_pivoting_fallback(o, w, [&] {
+        if (o.precision == RGF_PRECISION_FP64)
+            run_carry(c, w, o);
+        else
+            run_refined(c, w, o);
+    });
     write_smatrix(w, sel, S_out);
     if (info) {
         info->swept_right_to_left = w.mirrored;
@@ -1216,6 +1302,7 @@ void solve_one(rgf_context* c, const rgf_problem* p, const rgf_selection* s, con
         info->refine_residual = w.refine_residual;
         info->refine_s_err_estimate = w.refine_s_err;
         info->max_multiplier = w.max_multiplier;
+        info->pivoted = w.pivot_all;
         info->block_size = w.block_size;
         info->autotuned = w.tuned;
         info->factor_slices_resident = w.factors_resident;
@@ -1343,6 +1430,7 @@ rgf_status rgf_destroy(rgf_handle h) {
         h->bufs.clear();
         if (h->d_status) RGF_HIP(hipFree(h->d_status));
         if (h->d_scratch) RGF_HIP(hipFree(h->d_scratch));
+        if (h->rb) rocblas_ok(rocblas_destroy_handle((rocblas_handle)h->rb), "rocblas_destroy_handle");
         if (h->own) RGF_HIP(hipStreamDestroy(h->own));
         for (auto& e : h->ov.ev)
             if (e) RGF_HIP(hipEventDestroy(e));
@@ -1499,6 +1587,20 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
         std::vector<int32_t> eng, rest;
         for (int32_t q = 0; q < count; ++q) (batch_engine_takes(&p[q], o) ? eng : rest).push_back(q);
         if (!eng.empty()) solve_batched_engine(h, eng, p, s, o, S_out, info, sts, msgs);
+        // problems whose slice inverses the engine's kernels refused (pivoting
+        // inside each diagonal block only) run again on the worker contexts
+        // with the inverses pivoted over all rows
+        rgf_options opiv = o;
+        opiv.pivoting = 1;
+        std::vector<char> pivot_first((size_t)count, 0);
+        if (o.pivoting == 0)
+            for (int32_t q : eng)
+                if (sts[(size_t)q] == RGF_ERR_NUMERICAL || sts[(size_t)q] == RGF_ERR_NOT_CONVERGED) {
+                    pivot_first[(size_t)q] = 1;
+                    sts[(size_t)q] = RGF_OK;
+                    msgs[(size_t)q].clear();
+                    rest.push_back(q);
+                }
         const int32_t nrest = (int32_t)rest.size();
         const int want = o.batch_concurrency > 0 ? o.batch_concurrency : 8;
         const int nw = std::max(1, std::min<int>(want, nrest));
@@ -1541,7 +1643,8 @@ rgf_status rgf_solve_batched(rgf_handle h, int32_t count, const rgf_problem* p, 
             }
             for (int32_t k = next++; k < nrest; k = next++) {
                 const int32_t q = order[(size_t)k];
-                sts[(size_t)q] = guard(c, [&] { solve_one(c, &p[q], &s[q], opt, S_out[q], info ? &info[q] : nullptr); });
+                const rgf_options* oq = pivot_first[(size_t)q] ? &opiv : opt;
+                sts[(size_t)q] = guard(c, [&] { solve_one(c, &p[q], &s[q], oq, S_out[q], info ? &info[q] : nullptr); });
                 if (sts[(size_t)q] != RGF_OK) msgs[(size_t)q] = c->err;
             }
         };
@@ -1578,21 +1681,34 @@ rgf_status rgf_dense_inverse(rgf_handle h, int32_t precision, int64_t n, double*
             throw Error(RGF_ERR_INVALID_ARGUMENT, "RGF_DENSE_3M applies to the FP32 kernels");
         RGF_HIP(hipSetDevice(h->device));
         CMat<double> M = dmat<double>(h, "dense_A", n, n);
-        upload_colmajor(h, A, n, n, M);
-        reset_status(h);
-        if (precision == RGF_PRECISION_FP64) {
-            const int b = block_size > 0 ? block_size : default_block(precision);
-            double* ws = dbuf<double>(h, "inv_ws", inverse_workspace_elems<double>(n, b));
-            invert_inplace<double>(h->stream, n, M, b, ws, h->d_status, 0, nullptr, &h->ov);
-        } else {
-            const int b = block_size > 0 ? block_size : default_block(precision);
-            CMat<float> M32 = dmat<float>(h, "dense_A32", n, n);
-            convert<float, double>(h->stream, n, n, M, M32, 1.0);
-            float* ws = dbuf<float>(h, "inv_ws32", inverse_workspace_elems<float>(n, b));
-            invert_inplace<float>(h->stream, n, M32, b, ws, h->d_status, use3m, nullptr, &h->ov);
-            convert<double, float>(h->stream, n, n, M32, M, 1.0);
+        // the blocked kernels; a refused matrix is inverted again with
+        // partial pivoting over all rows, as the solves do
+        Work w;
+        w.pivot_all = block_size < 0;
+        for (;;) {
+            upload_colmajor(h, A, n, n, M);
+            reset_status(h);
+            if (precision == RGF_PRECISION_FP64) {
+                const int b = block_size > 0 ? block_size : default_block(precision);
+                double* ws = w.pivot_all ? nullptr : dbuf<double>(h, "inv_ws", inverse_workspace_elems<double>(n, b));
+                invert_slice<double>(h, w, h->stream, n, M, b, ws, 0, nullptr, &h->ov);
+            } else {
+                const int b = block_size > 0 ? block_size : default_block(precision);
+                CMat<float> M32 = dmat<float>(h, "dense_A32", n, n);
+                convert<float, double>(h->stream, n, n, M, M32, 1.0);
+                float* ws = w.pivot_all ? nullptr : dbuf<float>(h, "inv_ws32", inverse_workspace_elems<float>(n, b));
+                invert_slice<float>(h, w, h->stream, n, M32, b, ws, use3m, nullptr, &h->ov);
+                convert<double, float>(h->stream, n, n, M32, M, 1.0);
+            }
+            try {
+                check_status(h, true, precision == RGF_PRECISION_FP64);
+            } catch (const SliceRefused&) {
+                if (w.pivot_all) throw;
+                w.pivot_all = 1;
+                continue;
+            }
+            break;
         }
-        check_status(h, true, precision == RGF_PRECISION_FP64);
         download_colmajor(h, M, n, n, A);
     });
 }

@@ -565,7 +565,7 @@ void run_chunk(rgf_context* h, std::vector<HostProb*>& ps, const rgf_problem* p,
 }  // namespace
 
 bool batch_engine_takes(const rgf_problem* p, const rgf_options& o) {
-    if (o.precision != RGF_PRECISION_FP32_REFINED || o.batch_engine == 2) return false;
+    if (o.precision != RGF_PRECISION_FP32_REFINED || o.batch_engine == 2 || o.pivoting == 1) return false;
     if (!p || (p->ndim != 2 && p->ndim != 3)) return false;
     int64_t N = 1;
     for (int d = 0; d < p->ndim - 1; ++d) N *= std::max<int64_t>(p->dims[d], 0);

@@ -27,6 +27,7 @@ struct rgf_context {
     std::vector<rgf_context*> workers;  // worker contexts of rgf_solve_batched
     std::vector<hipEvent_t> timer_events;  // kept across solves (options.profile)
     void* scratch = nullptr;               // the device's no-allocation workspace
+    void* rb = nullptr;                    // dense-solver library handle (pivoted slice inverses)
 };
 
 namespace rgf {
@@ -92,6 +93,7 @@ struct Work {
     int64_t factors_resident = 0;  // mixed path: slice inverses kept resident
     int64_t factors_rebuilt = 0;   // mixed path: slice inverses rebuilt by the sweeps
     double* fields = nullptr;  // host: field profiles wanted (rgf_solve_fields)
+    int pivot_all = 0;         // 1: every slice inverted with partial pivoting over all rows
 };
 
 // Transverse mode tables f1 (n1 x n1) and f2 (n2 x n2), interleaved FP64,

@@ -189,6 +189,13 @@ void grad_contract(hipStream_t s, int64_t N, int64_t m, CMat<double> psi, CMat<d
 void* noalloc_scratch(int device);
 // p[0..n) = 0 with a kernel (n <= 64; capturable, unlike a memset node)
 void zero_words(hipStream_t s, int* p, int n);
+// planar (ld) <-> interleaved complex (ld = N) N x N row-major copies
+template <typename T>
+void to_interleaved(hipStream_t s, int64_t N, CMat<T> A, T* out);
+template <typename T>
+void from_interleaved(hipStream_t s, int64_t N, const T* in, CMat<T> A);
+// *status |= 1 when info[0] or info[1] is non-zero
+void fold_info(hipStream_t s, const int* info, int* status);
 // slice-inverse block size per precision (FP64 64, FP32 128)
 template <typename T>
 constexpr int diag_block_max() {

@@ -2837,6 +2837,44 @@ void zero_words(hipStream_t s, int* p, int n) {
     RGF_HIP(hipGetLastError());
 }
 
+// planar (ld) <-> interleaved complex (re, im pairs, ld = N), row-major N x N
+template <typename T>
+__global__ void to_interleaved_kernel(int64_t N, const T* re, const T* im, int64_t ld, T* out) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * N; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / N, j = e % N;
+        out[2 * e] = re[i * ld + j];
+        out[2 * e + 1] = im[i * ld + j];
+    }
+}
+template <typename T>
+__global__ void from_interleaved_kernel(int64_t N, const T* in, T* re, T* im, int64_t ld) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * N; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / N, j = e % N;
+        re[i * ld + j] = in[2 * e];
+        im[i * ld + j] = in[2 * e + 1];
+    }
+}
+template <typename T>
+void to_interleaved(hipStream_t s, int64_t N, CMat<T> A, T* out) {
+    hipLaunchKernelGGL((to_interleaved_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, A.re, A.im, A.ld, out);
+    RGF_HIP(hipGetLastError());
+}
+template <typename T>
+void from_interleaved(hipStream_t s, int64_t N, const T* in, CMat<T> A) {
+    hipLaunchKernelGGL((from_interleaved_kernel<T>), dim3(grid_for(N * N)), dim3(256), 0, s, N, in, A.re, A.im, A.ld);
+    RGF_HIP(hipGetLastError());
+}
+
+// bit 1 of the status word when either of two LAPACK-style info words is
+// non-zero (an exactly zero pivot of the pivoted factorisation)
+__global__ void fold_info_kernel(const int* info, int* status) {
+    if (threadIdx.x == 0 && (info[0] != 0 || info[1] != 0)) atomicOr(status, 1);
+}
+void fold_info(hipStream_t s, const int* info, int* status) {
+    hipLaunchKernelGGL(fold_info_kernel, dim3(1), dim3(64), 0, s, info, status);
+    RGF_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ instantiations
 #define RGF_INST(T)                                                                                          \
     template void cgemm<T>(hipStream_t, int64_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>, cd, int, int64_t, \
@@ -2851,7 +2889,9 @@ void zero_words(hipStream_t s, int* p, int n) {
     template void tune_fill<T>(hipStream_t, int64_t, CMat<T>, uint32_t);                                      \
     template void copy_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>);                               \
     template void accumulate<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<double>, double);                \
-    template void add_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>);
+    template void add_mat<T>(hipStream_t, int64_t, int64_t, CMat<T>, CMat<T>, CMat<T>);                      \
+    template void to_interleaved<T>(hipStream_t, int64_t, CMat<T>, T*);                                       \
+    template void from_interleaved<T>(hipStream_t, int64_t, const T*, CMat<T>);
 
 RGF_INST(double)
 RGF_INST(float)

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
     assert declared == set(L.SIGNATURES), declared ^ set(L.SIGNATURES)
-    assert lib.rgf_abi_version() == 4
+    assert lib.rgf_abi_version() == 5
 
 
 def test_status_strings_and_default_options():

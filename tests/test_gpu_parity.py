@@ -115,16 +115,38 @@ def _second_block_nearly_singular(rng, n=256, b=64, scale=1e-9):
     return B[P]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
-def test_slice_inverse_ill_conditioned_block_fails_loudly(precision):
-    # Pivoting is confined to the diagonal block: a block that needs
-    # pivoting and is ill-conditioned must be reported, not inverted silently
-    # with the growth of its multipliers (RGF_ERR_NUMERICAL).
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32_refined", 2e-5)])
+def test_slice_inverse_ill_conditioned_block_pivots_over_all_rows(precision, tol):
+    # The blocked kernels pivot inside the diagonal block only: this
+    # well-conditioned matrix, whose second 64-wide Schur-complement block is
+    # nearly singular, is refused by their growth check (not inverted silently
+    # with the growth of its multipliers) and inverted again with partial
+    # pivoting over all rows, as MATLAB's inv does (cal_smatrix_RGF.m:265).
     rng = np.random.default_rng(11)
     A = _second_block_nearly_singular(rng)
     assert np.linalg.cond(A) < 10
-    with pytest.raises(rgf_amd.RGFError, match="ill-conditioned diagonal block"):
-        rgf_amd.dense_inverse(A, precision, 64)
+    X = rgf_amd.dense_inverse(A, precision, 64)
+    ref = np.linalg.inv(A)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32_refined", 2e-5)])
+@pytest.mark.parametrize("n", [5, 200, 333])
+def test_pivoted_slice_inverse(precision, tol, n):
+    # the pivoted inverse on its own (block_size < 0), including a matrix
+    # whose leading entries are zero (row interchanges across the whole matrix)
+    rng = np.random.default_rng(n + 17)
+    A = _well_posed(rng, n)
+    A[[0, n - 1]] = A[[n - 1, 0]]
+    X = rgf_amd.dense_inverse(A, precision, -1)
+    ref = np.linalg.inv(A)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < tol
+
+
+def test_pivoted_slice_inverse_singular_is_refused():
+    A = np.ones((40, 40), dtype=complex)
+    with pytest.raises(rgf_amd.RGFError, match="zero or non-finite pivot"):
+        rgf_amd.dense_inverse(A, "fp64", -1)
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("fp32_refined", TOL_MIXED)])
