@@ -1773,13 +1773,17 @@ rgf_status rgf_dense_gemm(rgf_handle h, int32_t precision, int64_t m, int64_t n,
         CMat<double> dA = dmat<double>(h, "gA", m, k), dB = dmat<double>(h, "gB", k, n), dC = dmat<double>(h, "gC", m, n);
         upload_colmajor(h, A, m, k, dA);
         upload_colmajor(h, B, k, n, dB);
+        // with the solver's split-K workspace: thin products with a long inner
+        // dimension take the split and its reduction, as in the sweeps
         if (precision == RGF_PRECISION_FP64) {
-            cgemm<double>(h->stream, m, n, k, dA, dB, dC, cd(1, 0), 0);
+            const auto wk = splitk_ws<double>(h);
+            cgemm<double>(h->stream, m, n, k, dA, dB, dC, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
         } else {
             CMat<float> a = dmat<float>(h, "gA32", m, k), b = dmat<float>(h, "gB32", k, n), c = dmat<float>(h, "gC32", m, n);
             convert<float, double>(h->stream, m, k, dA, a, 1.0);
             convert<float, double>(h->stream, k, n, dB, b, 1.0);
-            cgemm<float>(h->stream, m, n, k, a, b, c, cd(1, 0), 0);
+            const auto wk = splitk_ws<float>(h);
+            cgemm<float>(h->stream, m, n, k, a, b, c, cd(1, 0), 0, 0, 0, 0, wk.first, wk.second);
             convert<double, float>(h->stream, m, n, c, dC, 1.0);
         }
         download_colmajor(h, dC, m, n, C);

@@ -26,7 +26,12 @@ def _crand(rng, *shape):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32_refined", 2e-6)])
 @pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200), (300, 257, 129),
-                                 (513, 1100, 96)])
+                                 (513, 1100, 96),
+                                 # thin products with a long inner dimension: split along K over the
+                                 # streamlined tiles (ragged M, N a multiple of 4) and, with N = 70
+                                 # (rows not 16-byte aligned), over the general tiles; both reduced
+                                 # by the split-K reduction kernel
+                                 (1000, 92, 2048), (4096, 184, 1024), (300, 70, 1500)])
 def test_mfma_complex_gemm(precision, tol, mnk):
     m, n, k = mnk
     rng = np.random.default_rng(m * 7 + n * 3 + k)
