@@ -940,28 +940,35 @@ __global__ __launch_bounds__(WM* WN * 64, (gemm_min_blocks<T, WM, WN, BK>())) vo
     cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g, lin);
 }
 
-// FP32 bulk update of the slice inverse: C = C + alpha A B with alpha = -1
-// or +1 (folded into A), C itself the beta = 1 source, three (M3) or four
+// Bulk update of the slice inverse: C = C + alpha A B with alpha = -1 or +1
+// (folded into A), C itself the beta = 1 source, three (M3, FP32) or four
 // real products, over the compressed tile grid (set_tiles), one 64 x 64
-// tile per work group.  The arithmetic is that of cgemm2_tile<float, 64, 64,
-// 16, 2, 2, false, 2, true, M3> (the same results, bit for bit); left out is
+// tile per work group.  The arithmetic is that of cgemm2_tile<T, 64, 64,
+// BK, 2, 2, false, 2, true, M3> (the same results, bit for bit); left out is
 // what the bulk update does not use (further problems in the launch, second
 // destination, split-K, hand-over, multiplier maximum), and C is read after
 // the last K chunk rather than beside it, the other groups on the CU
-// covering those reads.  At N = 4096, b = 128 it takes 111 us against 121
-// with three products and 137 against 146 with four.  (The tile loop lets a
-// group take several tiles; launched with one group per tile it keeps the
-// one-tile groups' turnover of CU slots, which the launches beside it need.)
-template <bool M3>
-__global__ __launch_bounds__(256, 4) void bulk_update_kernel(GemmArgs<float> g) {
-    using T = float;
-    using V = Mfma<T>::v4;
-    using VT = Vec<T>::type;
-    constexpr int BM = 64, BN = 64, BK = 16, WN = 2;
+// covering those reads.  At N = 4096 it takes, alone, 111 us against 121
+// (FP32, b = 128, three products), 75 against 87 (FP32, b = 64) and 162
+// against 176 (FP64, b = 64).  (The tile loop lets a group take several
+// tiles; launched with one group per tile it keeps the one-tile groups'
+// turnover of CU slots, which the launches beside it need.)
+template <typename T, bool M3>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void bulk_update_kernel(GemmArgs<T> g) {
+    using V = typename Mfma<T>::v4;
+    using VT = typename Vec<T>::type;
+    // K chunks of 16 (FP32) or 8 (FP64: 42 KB of LDS, 168 registers, three
+    // groups per CU); MFMA step k at lane group g4 takes chunk row KS g4 + k
+    constexpr int BM = 64, BN = 64, BK = sizeof(T) == 8 ? 8 : 16, WN = 2;
     constexpr int VW = Vec<T>::W;
-    constexpr int TM = 32, TN = 32, FM = 2, FN = 2, KS = 4;
+    constexpr int TM = 32, TN = 32, FM = 2, FN = 2, KS = BK / 4;
     constexpr int LA = BK + 4, LB = BN + 4;  // padded LDS rows (16-byte multiples)
     static_assert(BM * BK / VW == 256 && BK * BN / VW == 256, "one vector per thread and operand");
+    static_assert(KS == VW, "a lane's A values of a chunk are one vector");
+    static_assert(!M3 || sizeof(T) == 4, "three real products in FP32 only");
+    // B column feeding MFMA row i: FP64 accumulators hold rows g4 + 4r, which
+    // this permutation sends to columns 4 g4 + r
+    auto bperm = [](int i) { return sizeof(T) == 8 ? 4 * (i & 3) + (i >> 2) : i; };
     __shared__ __align__(16) T sA[2][2][BM][LA];
     __shared__ __align__(16) T sB[2][2][BK][LB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1035,7 +1042,7 @@ __global__ __launch_bounds__(256, 4) void bulk_update_kernel(GemmArgs<float> g) 
             }
 #pragma unroll
             for (int y = 0; y < FN; ++y) {
-                const int col = wn * TN + y * 16 + l16;
+                const int col = wn * TN + y * 16 + bperm(l16);
 #pragma unroll
                 for (int k = 0; k < KS; ++k) {
                     br[y][k] = sB[buf][0][KS * g4 + k][col];
@@ -1112,12 +1119,15 @@ __global__ __launch_bounds__(256, 4) void bulk_update_kernel(GemmArgs<float> g) 
                 if (gi >= M) continue;
                 const int64_t o = (int64_t)gi * ldc + gj;
                 if (vc && gj + 4 <= N) {
-                    const VT xr = *(const VT*)(Cre + o);
-                    const VT xi = *(const VT*)(Cim + o);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        cxr[x][y][r] = xr[r];
-                        cxi[x][y][r] = xi[r];
+                    for (int u = 0; u < 4; u += VW) {
+                        const VT xr = *(const VT*)(Cre + o + u);
+                        const VT xi = *(const VT*)(Cim + o + u);
+#pragma unroll
+                        for (int v = 0; v < VW; ++v) {
+                            cxr[x][y][u + v] = xr[v];
+                            cxi[x][y][u + v] = xi[v];
+                        }
                     }
                 } else {
 #pragma unroll
@@ -1152,14 +1162,17 @@ __global__ __launch_bounds__(256, 4) void bulk_update_kernel(GemmArgs<float> g) 
                     vi[r] += cxi[x][y][r];
                 }
                 if (whole) {
-                    VT xr, xi;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        xr[r] = vr[r];
-                        xi[r] = vi[r];
+                    for (int u = 0; u < 4; u += VW) {
+                        VT xr, xi;
+#pragma unroll
+                        for (int v = 0; v < VW; ++v) {
+                            xr[v] = vr[u + v];
+                            xi[v] = vi[u + v];
+                        }
+                        *(VT*)(Cre + o + u) = xr;
+                        *(VT*)(Cim + o + u) = xi;
                     }
-                    *(VT*)(Cre + o) = xr;
-                    *(VT*)(Cim + o) = xi;
                 } else {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
@@ -1334,26 +1347,32 @@ static bool launch_cplx(int use3m, hipStream_t s, const GemmArgs<T>& g0, const G
     return launch_pair2<T, BM, BN, BK, WM, WN, false>(s, g0, g1, g2, stop);
 }
 
-// FP32 bulk update (bulk_update_kernel), one work group per tile.  Returns
-// false (launching nothing) outside its preconditions: alpha = +-1, beta = 1
-// from C itself, no split-K, second destination, maximum or hand-over, K a
-// multiple of 16, 16-byte aligned operand rows.
-static bool launch_bulk_update(hipStream_t s, const GemmArgs<float>& g_, int use3m) {
-    constexpr int VW = Vec<float>::W;
-    const GemmArgs<float>& c = g_;
+// Bulk update of the slice inverse (bulk_update_kernel), one work group per
+// tile.  Returns false (launching nothing) outside its preconditions: alpha
+// = +-1, beta = 1 from C itself, no split-K, second destination, maximum or
+// hand-over, K a multiple of the K chunk (16 FP32, 8 FP64), three products
+// in FP32 only, 16-byte aligned operand rows.
+template <typename T>
+static bool launch_bulk_update(hipStream_t s, const GemmArgs<T>& g_, int use3m) {
+    constexpr int VW = Vec<T>::W, BK = sizeof(T) == 8 ? 8 : 16;
+    const GemmArgs<T>& c = g_;
     const bool unit = c.alpha_im == 0.0 && (c.alpha_re == 1.0 || c.alpha_re == -1.0);
     const int64_t nv = ((int64_t)c.N + VW - 1) / VW * VW;
     const bool ysec = c.Yre && c.y_rhi > c.y_rlo && c.y_chi > c.y_clo;
     if (!unit || c.beta != 1 || c.Xre != c.Cre || c.Xim != c.Cim || c.ldx != c.ldc || c.nz != 1 || c.K <= 0 ||
-        c.K % 16 != 0 || !c.vec_ok || nv > c.ldb || c.M <= 0 || c.N <= 0 || c.amax || c.ctr || c.ctr_wait || ysec)
+        c.K % BK != 0 || !c.vec_ok || nv > c.ldb || c.M <= 0 || c.N <= 0 || c.amax || c.ctr || c.ctr_wait || ysec)
         return false;
-    GemmArgs<float> g = g_;
-    set_tiles<float, 64, 64, true>(g);
+    if (use3m && sizeof(T) != 4) return false;
+    GemmArgs<T> g = g_;
+    set_tiles<T, 64, 64, true>(g);
     if (g.tiles <= 0) return true;
-    if (use3m)
-        hipLaunchKernelGGL(bulk_update_kernel<true>, dim3(g.tiles), dim3(256), 0, s, g);
-    else
-        hipLaunchKernelGGL(bulk_update_kernel<false>, dim3(g.tiles), dim3(256), 0, s, g);
+    if constexpr (sizeof(T) == 4) {
+        if (use3m) {
+            hipLaunchKernelGGL((bulk_update_kernel<T, true>), dim3(g.tiles), dim3(256), 0, s, g);
+            return true;
+        }
+    }
+    hipLaunchKernelGGL((bulk_update_kernel<T, false>), dim3(g.tiles), dim3(256), 0, s, g);
     return true;
 }
 
@@ -2524,7 +2543,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
                 gemm_args<T>(N, N, nb, Aold[q & 1], A.offset(p0, 0), A, cd(-1, 0), 1, Skip{p0, p2, p0, p2, p2, p3});
             constexpr int BK = GemmCfg<T>::BK2;
             bool done = false;
-            if constexpr (sizeof(T) == 4) done = launch_bulk_update(s2, g, use3m);
+            done = launch_bulk_update<T>(s2, g, use3m);
             if (!done)
                 done = use3m ? launch_pair2<T, 64, 64, BK, 2, 2, false, 2, true, true>(s2, g, nullptr)
                              : launch_pair2<T, 64, 64, BK, 2, 2, false, 2, true, false>(s2, g, nullptr);
