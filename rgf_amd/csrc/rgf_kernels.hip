@@ -940,21 +940,23 @@ __global__ __launch_bounds__(WM* WN * 64, (gemm_min_blocks<T, WM, WN, BK>())) vo
     cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g, lin);
 }
 
-// Bulk update of the slice inverse: C = C + alpha A B with alpha = -1 or +1
-// (folded into A), C itself the beta = 1 source, three (M3, FP32) or four
-// real products, over the compressed tile grid (set_tiles), one 64 x 64
-// tile per work group.  The arithmetic is that of cgemm2_tile<T, 64, 64,
-// BK, 2, 2, false, 2, true, M3> (the same results, bit for bit); left out is
-// what the bulk update does not use (further problems in the launch, second
-// destination, split-K, hand-over, multiplier maximum), and C is read after
-// the last K chunk rather than beside it, the other groups on the CU
-// covering those reads.  At N = 4096 it takes, alone, 111 us against 121
-// (FP32, b = 128, three products), 75 against 87 (FP32, b = 64) and 162
-// against 176 (FP64, b = 64).  (The tile loop lets a group take several
-// tiles; launched with one group per tile it keeps the one-tile groups'
-// turnover of CU slots, which the launches beside it need.)
-template <typename T, bool M3>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void bulk_update_kernel(GemmArgs<T> g) {
+// Lean single-problem tile kernel: C = alpha A B (+ C if BETA) with alpha =
+// -1 or +1 (folded into A), C itself the beta = 1 source, three (M3, FP32)
+// or four real products, 64 x 64 tiles over the (optionally compressed)
+// tile grid of set_tiles, one tile per work group; SPLIT: chunk z of an
+// nz-way split along K writes its partial to C + z * zstride.  The
+// arithmetic is that of cgemm2_tile<T, 64, 64, BK, 2, 2, false, MAP, true,
+// M3> (the same results, bit for bit); left out is what these products do
+// not use (further problems in the launch, second destination, hand-over,
+// multiplier maximum), and C is read after the last K chunk rather than
+// beside it, the other groups on the CU covering those reads.  The slice
+// inverse's bulk update at N = 4096 takes, alone, 111 us against 121 (FP32,
+// b = 128, three products), 75 against 87 (FP32, b = 64) and 162 against
+// 176 (FP64, b = 64).  (The tile loop lets a group take several tiles;
+// launched with one group per tile it keeps the one-tile groups' turnover
+// of CU slots, which the launches beside the bulk update need.)
+template <typename T, bool M3, bool BETA, bool SPLIT>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void lean_gemm_kernel(GemmArgs<T> g) {
     using V = typename Mfma<T>::v4;
     using VT = typename Vec<T>::type;
     // K chunks of 16 (FP32) or 8 (FP64: 42 KB of LDS, 168 registers, three
@@ -975,25 +977,30 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void bulk_update_kerne
     const int wm = wave / WN, wn = wave % WN;
     const int l16 = lane & 15, g4 = lane >> 4;
     const T sgn = (T)g.alpha_re;
-    const int M = g.M, N = g.N, nk = g.K / BK;
+    const int M = g.M, N = g.N;
     const int64_t lda = g.lda, ldb = g.ldb, ldc = g.ldc;
-    const T* __restrict__ Are = g.Are;
-    const T* __restrict__ Aim = g.Aim;
-    const T* __restrict__ Bre = g.Bre;
-    const T* __restrict__ Bim = g.Bim;
-    T* const Cre = g.Cre;
-    T* const Cim = g.Cim;
     const int NVW = (N + VW - 1) / VW * VW;
     const int tiles_n = g.tiles_n, ntiles = g.tiles;
+    const int per = SPLIT ? ntiles / g.nz : ntiles;
     const int cr_lo = g.cr_lo, cr_n = g.cr_n, cc_lo = g.cc_lo, cc_n = g.cc_n;
     const int srlo = g.srlo, srhi = g.srhi, sclo = g.sclo, schi = g.schi;
     const int sblo = g.sblo, sbhi = g.sbhi;
-    const bool vc = (ldc % 4) == 0 && (((uintptr_t)Cre | (uintptr_t)Cim) & 15) == 0;
+    const bool vc = (ldc % 4) == 0 && (((uintptr_t)g.Cre | (uintptr_t)g.Cim) & 15) == 0 && (g.zstride % 4) == 0;
     // the thread's operand vectors: A row ai_, columns akv; B row bk, columns bnv
     const int ai_ = tid / (BK / VW), akv = (tid % (BK / VW)) * VW;
     const int bk = tid / (BN / VW), bnv = (tid % (BN / VW)) * VW;
     for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
-        int tr = t / tiles_n, tc = t - tr * tiles_n;
+        // chunk z of the split: K range [z kc, z kc + kc) (the last one shorter)
+        const int z = SPLIT ? t / per : 0, lin = t - z * per;
+        const int64_t koff = (int64_t)z * g.kc;
+        const int nk = (SPLIT ? (int)min((int64_t)g.kc, (int64_t)g.K - koff) : g.K) / BK;
+        const T* __restrict__ Are = g.Are + koff;
+        const T* __restrict__ Aim = g.Aim + koff;
+        const T* __restrict__ Bre = g.Bre + koff * ldb;
+        const T* __restrict__ Bim = g.Bim + koff * ldb;
+        T* const Cre = g.Cre + z * g.zstride;
+        T* const Cim = g.Cim + z * g.zstride;
+        int tr = lin / tiles_n, tc = lin - tr * tiles_n;
         if (tr >= cr_lo) tr += cr_n;
         if (tc >= cc_lo) tc += cc_n;
         const int tm = tr * BM, tn = tc * BN;
@@ -1107,7 +1114,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void bulk_update_kerne
         // C += the tile (a lane holds C[row][4 g4 + r], r = 0..3), masked to
         // the rows and columns outside the skips; all of the lane's C reads
         // are issued before the first store
-        V cxr[FM][FN], cxi[FM][FN];
+        V cxr[BETA ? FM : 1][BETA ? FN : 1], cxi[BETA ? FM : 1][BETA ? FN : 1];
+        if constexpr (BETA)
 #pragma unroll
         for (int x = 0; x < FM; ++x)
 #pragma unroll
@@ -1158,8 +1166,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void bulk_update_kerne
                         vr[r] -= ii[x][y][r];
                         vi[r] -= rr[x][y][r] + ii[x][y][r];
                     }
-                    vr[r] += cxr[x][y][r];
-                    vi[r] += cxi[x][y][r];
+                    if constexpr (BETA) {
+                        vr[r] += cxr[x][y][r];
+                        vi[r] += cxi[x][y][r];
+                    }
                 }
                 if (whole) {
 #pragma unroll
@@ -1347,32 +1357,54 @@ static bool launch_cplx(int use3m, hipStream_t s, const GemmArgs<T>& g0, const G
     return launch_pair2<T, BM, BN, BK, WM, WN, false>(s, g0, g1, g2, stop);
 }
 
-// Bulk update of the slice inverse (bulk_update_kernel), one work group per
-// tile.  Returns false (launching nothing) outside its preconditions: alpha
-// = +-1, beta = 1 from C itself, no split-K, second destination, maximum or
-// hand-over, K a multiple of the K chunk (16 FP32, 8 FP64), three products
-// in FP32 only, 16-byte aligned operand rows.
-template <typename T>
-static bool launch_bulk_update(hipStream_t s, const GemmArgs<T>& g_, int use3m) {
+// One problem on lean_gemm_kernel, one work group per tile; COMPRESS: whole
+// skipped tile rows and columns left out of the grid.  Returns false
+// (launching nothing) outside its preconditions: alpha = +-1, beta = 0 or 1
+// from C itself, no second destination, maximum or hand-over, K (and the
+// split's chunk) a multiple of the K chunk (16 FP32, 8 FP64), three
+// products in FP32 only, 16-byte aligned operand rows.
+template <typename T, bool COMPRESS = false>
+static bool launch_lean(hipStream_t s, const GemmArgs<T>& g_, int use3m, hipEvent_t stop = nullptr) {
     constexpr int VW = Vec<T>::W, BK = sizeof(T) == 8 ? 8 : 16;
     const GemmArgs<T>& c = g_;
     const bool unit = c.alpha_im == 0.0 && (c.alpha_re == 1.0 || c.alpha_re == -1.0);
     const int64_t nv = ((int64_t)c.N + VW - 1) / VW * VW;
     const bool ysec = c.Yre && c.y_rhi > c.y_rlo && c.y_chi > c.y_clo;
-    if (!unit || c.beta != 1 || c.Xre != c.Cre || c.Xim != c.Cim || c.ldx != c.ldc || c.nz != 1 || c.K <= 0 ||
-        c.K % BK != 0 || !c.vec_ok || nv > c.ldb || c.M <= 0 || c.N <= 0 || c.amax || c.ctr || c.ctr_wait || ysec)
+    const bool src_c = c.Xre == c.Cre && c.Xim == c.Cim && c.ldx == c.ldc;
+    const bool split = c.nz > 1;
+    if (!unit || (c.beta != 0 && c.beta != 1) || (c.beta && (!src_c || split)) || c.K <= 0 || c.K % BK != 0 ||
+        (split && c.kc % BK != 0) || !c.vec_ok || nv > c.ldb || c.M <= 0 || c.N <= 0 || c.amax || c.ctr ||
+        c.ctr_wait || ysec || (use3m && sizeof(T) != 4))
         return false;
-    if (use3m && sizeof(T) != 4) return false;
     GemmArgs<T> g = g_;
-    set_tiles<T, 64, 64, true>(g);
-    if (g.tiles <= 0) return true;
+    set_tiles<T, 64, 64, COMPRESS>(g);
+    if (g.tiles <= 0) {
+        if (stop) RGF_HIP(hipEventRecord(stop, s));
+        return true;
+    }
+    auto go = [&](auto kern) {
+        if (stop)
+            hipExtLaunchKernelGGL(kern, dim3(g.tiles), dim3(256), 0, s, nullptr, stop, 0, g);
+        else
+            hipLaunchKernelGGL(kern, dim3(g.tiles), dim3(256), 0, s, g);
+    };
     if constexpr (sizeof(T) == 4) {
         if (use3m) {
-            hipLaunchKernelGGL((bulk_update_kernel<T, true>), dim3(g.tiles), dim3(256), 0, s, g);
+            if (split)
+                go(lean_gemm_kernel<T, true, false, true>);
+            else if (c.beta)
+                go(lean_gemm_kernel<T, true, true, false>);
+            else
+                go(lean_gemm_kernel<T, true, false, false>);
             return true;
         }
     }
-    hipLaunchKernelGGL((bulk_update_kernel<T, false>), dim3(g.tiles), dim3(256), 0, s, g);
+    if (split)
+        go(lean_gemm_kernel<T, false, false, true>);
+    else if (c.beta)
+        go(lean_gemm_kernel<T, false, true, false>);
+    else
+        go(lean_gemm_kernel<T, false, false, false>);
     return true;
 }
 
@@ -1441,7 +1473,7 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
                 g.nz = (int)nz;
                 g.kc = (int)kc;
                 g.zstride = per;
-                if (!launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g))
+                if (!launch_lean<T>(s, g, use3m) && !launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g))
                     launch_pair<T, 64, 64, GemmCfg<T>::BK_SMALL, 2, 2>(s, g, nullptr, use3m);
                 hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3(grid_for(M * N)), dim3(256), 0, s, M, N, (int)nz,
                                    (const T*)ws, per, C.re, C.im, C.ld, alpha.real(), alpha.imag(), beta);
@@ -1461,7 +1493,8 @@ void cgemm_skip(hipStream_t s, int64_t M, int64_t N, int64_t K, CMat<T> A, CMat<
         // streamlined kernel: 64x64 tiles (4 waves, K chunk 16) when there
         // are enough tiles, else 32x32
         const GemmArgs<T> g = gemm_args<T>(M, N, K, A, B, C, alpha, beta, sk);
-        const bool done = tiles_mid >= 256 ? launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g)
+        const bool done = tiles_mid >= 256 ? (launch_lean<T>(s, g, use3m) ||
+                                              launch_cplx<T, 64, 64, GemmCfg<T>::BK2, 2, 2>(use3m, s, g))
                                            : launch_cplx<T, 32, 32, 16, 1, 1>(use3m, s, g);
         if (done) {
             RGF_HIP(hipGetLastError());
@@ -1556,7 +1589,7 @@ void sweep_forward(hipStream_t s, int64_t N, int64_t m, CMat<float> G, CMat<floa
         g.nz = (int)nz;
         g.kc = (int)kc;
         g.zstride = per;
-        if (launch_cplx<float, 64, 64, 16, 2, 2>(use3m, s, g)) {
+        if (launch_lean<float>(s, g, use3m) || launch_cplx<float, 64, 64, 16, 2, 2>(use3m, s, g)) {
             hipLaunchKernelGGL(sweep_reduce_kernel, dim3(grid_for(N * m)), dim3(256), 0, s, N, m, (int)nz,
                                (const float*)ws, per, D.re, D.im, D.ld, next ? next->re : nullptr,
                                next ? next->im : nullptr, next ? next->ld : 0, sc, t.re, t.im, t.ld);
@@ -2543,7 +2576,7 @@ void invert_inplace(hipStream_t s, int64_t N, CMat<T> A, int b, T* ws, int* dev_
                 gemm_args<T>(N, N, nb, Aold[q & 1], A.offset(p0, 0), A, cd(-1, 0), 1, Skip{p0, p2, p0, p2, p2, p3});
             constexpr int BK = GemmCfg<T>::BK2;
             bool done = false;
-            done = launch_bulk_update<T>(s2, g, use3m);
+            done = launch_lean<T, true>(s2, g, use3m);
             if (!done)
                 done = use3m ? launch_pair2<T, 64, 64, BK, 2, 2, false, 2, true, true>(s2, g, nullptr)
                              : launch_pair2<T, 64, 64, BK, 2, 2, false, 2, true, false>(s2, g, nullptr);
