@@ -159,7 +159,7 @@ def roofline(info: dict, prec: str, use3m: bool = False, kernel: str | None = No
             "algorithmic_bytes": pmc["algorithmic_bytes"] if pmc else None,
             "mfma_util_pct": pmc["mfma_util_pct"] if pmc else None,
             "counters_from": os.path.relpath(PMC_SUMMARY, ROOT) if pmc else None,
-            "kernel": kernel or f"cgemm trailing update of the slice inverse ({prec}{', 3 real products' if use3m else ''})",
+            "kernel": kernel or f"bulk (trailing) update of the slice inverse, lean_gemm_kernel ({prec}{', 3 real products' if use3m else ''})",
             "launches_timed": int(info["kernel_launches"]),
             "avg_us": round(1e6 * t / info["kernel_launches"], 2),
             "flops_per_launch": executed / info["kernel_launches"]}
