@@ -1889,6 +1889,9 @@ __device__ __forceinline__ void gj_diag_blk(T* Are, T* Aim, int64_t lda, int p0,
         // Invert the pivot tile of panel q (in xr[q], up to date) into dr/di;
         // publish D_q, max|D_q|^2 and the verdict of the elimination checks.
         auto factor = [&](int q) {
+            // the elimination is the panel's critical path: its wave takes
+            // issue priority over the tile-update wave sharing its SIMD
+            __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int bi = 0; bi < NW; ++bi)
                 if (bi == q) {
@@ -1918,6 +1921,7 @@ __device__ __forceinline__ void gj_diag_blk(T* Are, T* Aim, int64_t lda, int p0,
                 okp[q] = (!bad && grow_ok) ? 1 : 0;
                 dmax[q] = wd;
             }
+            __builtin_amdgcn_s_setprio(0);
         };
         if (wave == 0) {
             float m = 0.f;
