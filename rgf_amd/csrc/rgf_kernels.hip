@@ -806,8 +806,8 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
             stash(buf ^ 1);
             __syncthreads();
         }
-        if (beta) load_c();
         chunk((nk - 1) & 1);
+        if (beta) load_c();
     } else {
         for (int kc = 0; kc < nk; ++kc) {
             const int buf = kc & 1;
