@@ -597,10 +597,11 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
     // operand column feeding MFMA row i is bperm(i) (FP64 accumulators hold
     // rows g4 + 4r, which the permutation sends to columns 4 g4 + r).
     auto bperm = [](int i) { return (TR && sizeof(T) == 8) ? 4 * (i & 3) + (i >> 2) : i; };
-    // TR: the beta = 1 source is read at the start of the last K chunk, into
-    // cx (its load latency hidden behind that chunk's MFMAs), and added in the
-    // epilogue; the accumulators start from zero, so the entries of C are
-    // rounded once per launch instead of once per K step.
+    // TR: the beta = 1 source is read after the last K chunk, into cx (the
+    // other groups on the CU cover the load latency; reading it beside the
+    // last chunk's MFMAs measured 1 % slower in the slice inverse), and added
+    // in the epilogue; the accumulators start from zero, so the entries of C
+    // are rounded once per launch instead of once per K step.
     constexpr bool LATE = TR;
     V cxr[LATE ? FM : 1][LATE ? FN : 1], cxi[LATE ? FM : 1][LATE ? FN : 1];
     // (the lambda reads the source through locals: kernel arguments taken
@@ -796,7 +797,7 @@ __device__ __forceinline__ void cgemm2_tile(const GemmArgs<T>& g, int lin) {
         }
     };
     if constexpr (LATE) {
-        // the last chunk peeled off: no prefetch there, the C reads instead
+        // the last chunk peeled off: no prefetch there, the C reads after it
         for (int kc = 0; kc + 1 < nk; ++kc) {
             const int buf = kc & 1;
             fetch((kc + 1) * BK);
