@@ -935,10 +935,18 @@ __global__ __launch_bounds__(WM* WN * 64, (gemm_min_blocks<T, WM, WN, BK>())) vo
         const int q = total >> 3, rem = total & 7, x = bid & 7, loc = bid >> 3;
         bid = x * q + min(x, rem) + loc;
     }
-    const int which = bid < g0.tiles ? 0 : (bid < g0.tiles + g1.tiles ? 1 : 2);
-    const GemmArgs<T>& g = which == 0 ? g0 : (which == 1 ? g1 : g2);
-    const int lin = bid - (which > 0 ? g0.tiles : 0) - (which > 1 ? g1.tiles : 0);
-    cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g, lin);
+    // one inlined copy of the tile per problem: each reads its own arguments
+    // at fixed offsets instead of selecting among the three per field
+    if (bid < g0.tiles) {
+        cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g0, bid);
+        return;
+    }
+    bid -= g0.tiles;
+    if (bid < g1.tiles) {
+        cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g1, bid);
+        return;
+    }
+    cgemm2_tile<T, BM, BN, BK, WM, WN, ACC3, MAP, TR, M3, AMAX>(g2, bid - g1.tiles);
 }
 
 // Lean single-problem tile kernel: C = alpha A B (+ C if BETA) with alpha =
